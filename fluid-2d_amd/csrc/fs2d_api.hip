@@ -1,0 +1,434 @@
+// fs2d_api.hip -- the C ABI of libfs2d_hip.so (declared in include/fs2d_hip.h).
+//
+// A handle owns its HIP stream and every device field of one simulation.
+// Fields are allocated once at create time with a 256-byte row pitch and stay
+// resident in HBM; the step composes the stage launchers in the order of
+// fluid_solver_cpu::solve (src/fluid_solver_cpu.cpp:6-31) and swaps buffer
+// pointers instead of copying where the reference copies a grid
+// (initial_grid in advect, temp_* before the velocity advection).
+#include "../../include/fs2d_hip.h"
+#include "fs2d_internal.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+
+using namespace fs2d;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+enum { F_RHO, F_U, F_V, F_RS, F_US, F_VS, F_X0, F_T1, F_T2, F_D, F_P, F_SCRATCH, F_COUNT };
+
+}  // namespace
+
+struct fs2d_handle {
+    int R = 0, P = 0;
+    size_t field_bytes = 0;
+    fs2d_params prm{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float* f[F_COUNT] = {};
+    unsigned* max_bits = nullptr;
+    std::map<int, GsPlan> gs_plans;  // by sweep count
+    int gs_tw = 64;
+    int jacobi_block = 4;
+    bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
+    std::string err;
+
+    float*& rho() { return f[F_RHO]; }
+    float*& u() { return f[F_U]; }
+    float*& v() { return f[F_V]; }
+};
+
+namespace {
+
+int fail(fs2d_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define FS2D_HIP(h, expr)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((h), FS2D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Scalars exactly as the reference computes them.
+struct Scalars {
+    float dt0, h, neg_half_h;
+};
+
+float cells(int R) { return static_cast<float>(static_cast<size_t>(R) * static_cast<size_t>(R)); }
+
+Scalars scalars(int R, float dt) {
+    Scalars s;
+    s.dt0 = std::sqrt(cells(R)) * dt;       // src/fluid_solver_cpu.cpp:126
+    s.h = 1.0f / std::sqrt(cells(R));       // :183
+    s.neg_half_h = -0.5f * s.h;             // :190, (-0.5f * h) * (...)
+    return s;
+}
+
+hipError_t gs_plan_for(fs2d_handle* h, int K, const GsPlan** out) {
+    auto it = h->gs_plans.find(K);
+    if (it == h->gs_plans.end()) {
+        GsPlan plan{};
+        hipError_t e = gs_plan_create(&plan, h->R, K, h->gs_tw);
+        if (e != hipSuccess) {
+            gs_plan_destroy(&plan);
+            return e;
+        }
+        it = h->gs_plans.emplace(K, plan).first;
+    }
+    *out = &it->second;
+    return hipSuccess;
+}
+
+// K relaxation sweeps of x against b; Jacobi may leave the result in the
+// scratch field, in which case the two pointers are swapped.
+int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, float denom, int mode, int K,
+                 int zero_start) {
+    if (K <= 0) return FS2D_OK;
+    SweepArgs A{x, b, h->R, h->P, kind, a, denom, ring_of(mode), K, zero_start};
+    if (h->prm.sweep_order == FS2D_ORDER_GS_LEX) {
+        const GsPlan* plan = nullptr;
+        FS2D_HIP(h, gs_plan_for(h, K, &plan));
+        FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
+    } else {
+        FS2D_HIP(h, launch_jacobi_solve(h->stream, A, h->f[F_SCRATCH], h->jacobi_block));
+        if (jacobi_launches(K, h->jacobi_block) & 1) std::swap(x, h->f[F_SCRATCH]);
+    }
+    return FS2D_OK;
+}
+
+int diffuse(fs2d_handle* h, float*& x, int mode, float rate, float dt, int K) {
+    // a = (dt * float(R*R)) * rate, src/fluid_solver_cpu.cpp:100
+    const float a = dt * cells(h->R) * rate;
+    const float denom = 1.f + 4.f * a;
+    return linear_solve(h, x, h->f[F_X0], SWEEP_DIFFUSE, a, denom, mode, K, 0);
+}
+
+int project(fs2d_handle* h, int K) {
+    const Scalars s = scalars(h->R, 1.0f);
+    FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h));
+    int rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1);
+    if (rc) return rc;
+    h->pressure_zero = (K <= 0);
+    FS2D_HIP(h, launch_gradient(h->stream, h->u(), h->v(), h->f[F_P], h->R, h->P, s.h, h->pressure_zero ? 1 : 0));
+    return FS2D_OK;
+}
+
+int advect_density(fs2d_handle* h, float dt) {
+    const Scalars s = scalars(h->R, dt);
+    FS2D_HIP(h, launch_fill(h->stream, h->max_bits, 0u, 1));
+    FS2D_HIP(h, launch_max_speed(h->stream, h->max_bits, h->u(), h->v(), h->R, h->P));
+    FS2D_HIP(h, launch_scatter_density(h->stream, h->f[F_T1], h->rho(), h->u(), h->v(), h->R, h->P, s.dt0,
+                                       h->max_bits));
+    std::swap(h->rho(), h->f[F_T1]);
+    return FS2D_OK;
+}
+
+int advect_velocity(fs2d_handle* h, float dt) {
+    const Scalars s = scalars(h->R, dt);
+    FS2D_HIP(h, launch_gather_velocity(h->stream, h->f[F_T1], h->f[F_T2], h->u(), h->v(), h->R, h->P, s.dt0));
+    std::swap(h->u(), h->f[F_T1]);
+    std::swap(h->v(), h->f[F_T2]);
+    return FS2D_OK;
+}
+
+// fluid_solver_cpu::solve, src/fluid_solver_cpu.cpp:6-31.
+int step(fs2d_handle* h, float diff, float visc, float dt) {
+    const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
+    const int R = h->R, P = h->P;
+    int rc;
+    // density: add source, diffuse, scatter-advect (:16-18)
+    FS2D_HIP(h, launch_add_sources(h->stream, h->rho(), h->f[F_X0], h->f[F_RS], R, P, dt));
+    if ((rc = diffuse(h, h->rho(), BND_CONT, diff, dt, Kd))) return rc;
+    if ((rc = advect_density(h, dt))) return rc;
+    // velocity: add sources, diffuse (:21-24; u and v are independent, so
+    // each field's add is followed directly by its diffuse)
+    FS2D_HIP(h, launch_add_sources(h->stream, h->u(), h->f[F_X0], h->f[F_US], R, P, dt));
+    if ((rc = diffuse(h, h->u(), BND_OPP_H, visc, dt, Kd))) return rc;
+    FS2D_HIP(h, launch_add_sources(h->stream, h->v(), h->f[F_X0], h->f[F_VS], R, P, dt));
+    if ((rc = diffuse(h, h->v(), BND_OPP_V, visc, dt, Kd))) return rc;
+    // project, self-advect, project (:25-30)
+    if ((rc = project(h, Kp))) return rc;
+    if ((rc = advect_velocity(h, dt))) return rc;
+    return project(h, Kp);
+}
+
+int copy_in(fs2d_handle* h, float* dst, const float* src, hipMemcpyKind kind) {
+    const size_t row = sizeof(float) * h->R;
+    FS2D_HIP(h, hipMemcpy2DAsync(dst, sizeof(float) * h->P, src, row, row, h->R, kind, h->stream));
+    return FS2D_OK;
+}
+
+int copy_out(fs2d_handle* h, float* dst, const float* src, hipMemcpyKind kind) {
+    const size_t row = sizeof(float) * h->R;
+    FS2D_HIP(h, hipMemcpy2DAsync(dst, row, src, sizeof(float) * h->P, row, h->R, kind, h->stream));
+    return FS2D_OK;
+}
+
+bool field_ok(int field) { return field >= FS2D_FIELD_DENSITY && field <= FS2D_FIELD_VELOCITY_V; }
+
+}  // namespace
+
+extern "C" {
+
+int fs2d_hip_abi_version(void) { return FS2D_ABI_VERSION; }
+
+void fs2d_default_params(fs2d_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->diffuse_iterations = 20;
+    p->pressure_iterations = 20;
+    p->sweep_order = FS2D_ORDER_GS_LEX;
+}
+
+int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle** out) {
+    if (!out) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: out is NULL");
+    *out = nullptr;
+    if (rows != cols)
+        return fail(nullptr, FS2D_ERR_ARGUMENT,
+                    "fs2d_hip_create: rows != cols; the reference solver is defined for square grids only "
+                    "(grid::cols() returns rows)");
+    if (rows < 3 || rows > 32768)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: rows must be in [3, 32768] (ring included)");
+    fs2d_params prm;
+    fs2d_default_params(&prm);
+    if (params) prm = *params;
+    if (prm.diffuse_iterations < 0 || prm.pressure_iterations < 0 || prm.diffuse_iterations > 1000000 ||
+        prm.pressure_iterations > 1000000)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: iteration counts must be in [0, 1e6]");
+    if (prm.sweep_order != FS2D_ORDER_GS_LEX && prm.sweep_order != FS2D_ORDER_JACOBI)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: unknown sweep_order");
+    if (prm.jacobi_block < 0 || prm.jacobi_block > 8)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: jacobi_block must be in [0, 8]");
+    if (prm.gs_tile_cols != 0 && prm.gs_tile_cols != 64 && prm.gs_tile_cols != 128)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_tile_cols must be 0, 64 or 128");
+    for (int r : prm.reserved)
+        if (r != 0) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: reserved fields must be 0");
+
+    fs2d_handle* h = new fs2d_handle();
+    h->R = rows;
+    h->P = (rows + 63) / 64 * 64;
+    h->field_bytes = sizeof(float) * static_cast<size_t>(h->R) * h->P;
+    h->prm = prm;
+    h->gs_tw = prm.gs_tile_cols ? prm.gs_tile_cols : 64;
+    h->jacobi_block = prm.jacobi_block ? prm.jacobi_block : 4;
+    auto bail = [&](hipError_t e, const char* what) {
+        std::string msg = std::string("fs2d_hip_create: ") + what + ": " + hipGetErrorString(e);
+        fs2d_hip_destroy(h);
+        return fail(nullptr, FS2D_ERR_HIP, msg);
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&h->device)) != hipSuccess) return bail(e, "hipGetDevice");
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "hipStreamCreate");
+    if ((e = hipEventCreate(&h->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&h->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
+    for (int i = 0; i < F_COUNT; ++i) {
+        if ((e = hipMalloc(&h->f[i], h->field_bytes)) != hipSuccess) return bail(e, "hipMalloc(field)");
+        if ((e = launch_fill(h->stream, h->f[i], 0u, h->field_bytes / 4)) != hipSuccess)
+            return bail(e, "zero fill");
+    }
+    if ((e = hipMalloc(&h->max_bits, 256)) != hipSuccess) return bail(e, "hipMalloc(scalar)");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
+    *out = h;
+    return FS2D_OK;
+}
+
+int fs2d_hip_destroy(fs2d_handle* h) {
+    if (!h) return FS2D_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto& kv : h->gs_plans) gs_plan_destroy(&kv.second);
+    for (float* p : h->f)
+        if (p) (void)hipFree(p);
+    if (h->max_bits) (void)hipFree(h->max_bits);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FS2D_OK;
+}
+
+const char* fs2d_hip_last_error(const fs2d_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+void* fs2d_hip_stream(fs2d_handle* h) { return h ? static_cast<void*>(h->stream) : nullptr; }
+
+int fs2d_hip_upload(fs2d_handle* h, const float* rho, const float* u, const float* v) {
+    if (!h || !rho || !u || !v) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_upload: NULL argument");
+    int rc;
+    if ((rc = copy_in(h, h->rho(), rho, hipMemcpyHostToDevice))) return rc;
+    if ((rc = copy_in(h, h->u(), u, hipMemcpyHostToDevice))) return rc;
+    if ((rc = copy_in(h, h->v(), v, hipMemcpyHostToDevice))) return rc;
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    return FS2D_OK;
+}
+
+int fs2d_hip_upload_device(fs2d_handle* h, const float* rho, const float* u, const float* v) {
+    if (!h || !rho || !u || !v) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_upload_device: NULL argument");
+    int rc;
+    if ((rc = copy_in(h, h->rho(), rho, hipMemcpyDeviceToDevice))) return rc;
+    if ((rc = copy_in(h, h->u(), u, hipMemcpyDeviceToDevice))) return rc;
+    return copy_in(h, h->v(), v, hipMemcpyDeviceToDevice);
+}
+
+int fs2d_hip_set_sources(fs2d_handle* h, const float* rs, const float* us, const float* vs) {
+    if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_set_sources: NULL handle");
+    const float* src[3] = {rs, us, vs};
+    const int dst[3] = {F_RS, F_US, F_VS};
+    for (int i = 0; i < 3; ++i) {
+        if (src[i]) {
+            int rc = copy_in(h, h->f[dst[i]], src[i], hipMemcpyHostToDevice);
+            if (rc) return rc;
+        } else {
+            FS2D_HIP(h, launch_fill(h->stream, h->f[dst[i]], 0u, h->field_bytes / 4));
+        }
+    }
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    return FS2D_OK;
+}
+
+int fs2d_hip_step_n(fs2d_handle* h, int n, float diff, float visc, float dt) {
+    if (!h || n < 0) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_step_n: bad argument");
+    for (int i = 0; i < n; ++i) {
+        int rc = step(h, diff, visc, dt);
+        if (rc) return rc;
+    }
+    return FS2D_OK;
+}
+
+int fs2d_hip_sync(fs2d_handle* h) {
+    if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_sync: NULL handle");
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    return FS2D_OK;
+}
+
+int fs2d_hip_download(fs2d_handle* h, float* rho, float* u, float* v) {
+    if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_download: NULL handle");
+    int rc;
+    if (rho && (rc = copy_out(h, rho, h->rho(), hipMemcpyDeviceToHost))) return rc;
+    if (u && (rc = copy_out(h, u, h->u(), hipMemcpyDeviceToHost))) return rc;
+    if (v && (rc = copy_out(h, v, h->v(), hipMemcpyDeviceToHost))) return rc;
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    return FS2D_OK;
+}
+
+int fs2d_hip_download_device(fs2d_handle* h, float* rho, float* u, float* v) {
+    if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_download_device: NULL handle");
+    int rc;
+    if (rho && (rc = copy_out(h, rho, h->rho(), hipMemcpyDeviceToDevice))) return rc;
+    if (u && (rc = copy_out(h, u, h->u(), hipMemcpyDeviceToDevice))) return rc;
+    if (v && (rc = copy_out(h, v, h->v(), hipMemcpyDeviceToDevice))) return rc;
+    return FS2D_OK;
+}
+
+int fs2d_hip_download_pressure(fs2d_handle* h, float* p) {
+    if (!h || !p) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_download_pressure: NULL argument");
+    if (h->pressure_zero) {
+        std::memset(p, 0, sizeof(float) * static_cast<size_t>(h->R) * h->R);
+        return FS2D_OK;
+    }
+    int rc = copy_out(h, p, h->f[F_P], hipMemcpyDeviceToHost);
+    if (rc) return rc;
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    return FS2D_OK;
+}
+
+int fs2d_hip_solve(fs2d_handle* h, float* rho, const float* rho_src, float diff, float* u, float* v,
+                   const float* u_src, const float* v_src, float visc, float dt) {
+    if (!h || !rho || !rho_src || !u || !v || !u_src || !v_src)
+        return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_solve: NULL argument");
+    int rc;
+    if ((rc = fs2d_hip_upload(h, rho, u, v))) return rc;
+    if ((rc = fs2d_hip_set_sources(h, rho_src, u_src, v_src))) return rc;
+    if ((rc = step(h, diff, visc, dt))) return rc;
+    return fs2d_hip_download(h, rho, u, v);
+}
+
+// ---- stages ------------------------------------------------------------------
+static float*& field_ref(fs2d_handle* h, int field) {
+    return field == FS2D_FIELD_DENSITY ? h->rho() : (field == FS2D_FIELD_VELOCITY_U ? h->u() : h->v());
+}
+
+int fs2d_hip_stage_add_sources(fs2d_handle* h, int field, float dt) {
+    if (!h || !field_ok(field)) return fail(h, FS2D_ERR_ARGUMENT, "stage_add_sources: bad argument");
+    const int src = field == FS2D_FIELD_DENSITY ? F_RS : (field == FS2D_FIELD_VELOCITY_U ? F_US : F_VS);
+    FS2D_HIP(h, launch_add_sources(h->stream, field_ref(h, field), nullptr, h->f[src], h->R, h->P, dt));
+    return FS2D_OK;
+}
+
+int fs2d_hip_stage_diffuse(fs2d_handle* h, int field, float rate, float dt, int iterations) {
+    if (!h || !field_ok(field) || iterations < 0)
+        return fail(h, FS2D_ERR_ARGUMENT, "stage_diffuse: bad argument");
+    float*& x = field_ref(h, field);
+    FS2D_HIP(h, hipMemcpyAsync(h->f[F_X0], x, h->field_bytes, hipMemcpyDeviceToDevice, h->stream));
+    return diffuse(h, x, field, rate, dt, iterations);
+}
+
+int fs2d_hip_stage_advect_density(fs2d_handle* h, float dt) {
+    if (!h) return fail(h, FS2D_ERR_ARGUMENT, "stage_advect_density: NULL handle");
+    return advect_density(h, dt);
+}
+
+int fs2d_hip_stage_advect_velocity(fs2d_handle* h, float dt) {
+    if (!h) return fail(h, FS2D_ERR_ARGUMENT, "stage_advect_velocity: NULL handle");
+    return advect_velocity(h, dt);
+}
+
+int fs2d_hip_stage_project(fs2d_handle* h, int iterations) {
+    if (!h || iterations < 0) return fail(h, FS2D_ERR_ARGUMENT, "stage_project: bad argument");
+    return project(h, iterations);
+}
+
+int fs2d_hip_stage_set_boundary(fs2d_handle* h, int field) {
+    if (!h || !field_ok(field)) return fail(h, FS2D_ERR_ARGUMENT, "stage_set_boundary: bad argument");
+    FS2D_HIP(h, launch_set_boundary(h->stream, field_ref(h, field), h->R, h->P, ring_of(field)));
+    return FS2D_OK;
+}
+
+// ---- measurement -------------------------------------------------------------
+int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
+    if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_pressure_sweeps: bad argument");
+    const Scalars s = scalars(h->R, 1.0f);
+    const int K = h->prm.pressure_iterations;
+    FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h));
+    int rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1);  // warm
+    if (rc) return rc;
+    FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < reps; ++i)
+        if ((rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1))) return rc;
+    FS2D_HIP(h, hipEventRecord(h->ev1, h->stream));
+    FS2D_HIP(h, hipEventSynchronize(h->ev1));
+    float total = 0.f;
+    FS2D_HIP(h, hipEventElapsedTime(&total, h->ev0, h->ev1));
+    *ms = total / reps;
+    return FS2D_OK;
+}
+
+int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
+    if (!h || !launches) return fail(h, FS2D_ERR_ARGUMENT, "launches_per_step: bad argument");
+    const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
+    auto solve_launches = [&](int K) -> int {
+        if (K <= 0) return 0;
+        if (h->prm.sweep_order == FS2D_ORDER_JACOBI) return jacobi_launches(K, h->jacobi_block);
+        const GsPlan* plan = nullptr;
+        if (gs_plan_for(h, K, &plan) != hipSuccess) return -1;
+        return plan->levels;
+    };
+    const int d = solve_launches(Kd), p = solve_launches(Kp);
+    if (d < 0 || p < 0) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
+    // add x3, diffuse x3, fill + max + scatter, project x2 (div + solve + grad), gather
+    *launches = 3 + 3 * d + 3 + 2 * (2 + p) + 1;
+    return FS2D_OK;
+}
+
+}  // extern "C"

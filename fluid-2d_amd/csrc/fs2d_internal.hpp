@@ -1,0 +1,80 @@
+// fs2d_internal.hpp -- shared declarations of the HIP stable-fluids solver.
+//
+// Device layout: every field is an R x R float grid (ring included) stored
+// row-major with a row pitch of P floats, P = R rounded up to 64 (256 bytes),
+// so each row starts on a 256-byte boundary.  Cell (i, j) is at p[i * P + j].
+// All arithmetic is float32 with FP contraction off and IEEE division, in the
+// evaluation order of src/fluid_solver_cpu.cpp; scalar constants (a, 1 + 4a,
+// dt0, h) are computed once on the host exactly as the reference computes them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fs2d {
+
+// Boundary rule of a field (src/fluid_solver_cpu.cpp:33-83), as the signs of
+// its top/bottom rows (tb) and left/right columns (lr) relative to the
+// adjacent interior cell:
+//   continuous  tb = +1, lr = +1   (density, pressure)
+//   opposite_h  tb = +1, lr = -1   (u)
+//   opposite_v  tb = -1, lr = +1   (v)
+// Corners are 0.5f * (tb(x) + lr(x)) of the diagonal interior cell x.
+enum Bnd : int { BND_CONT = 0, BND_OPP_H = 1, BND_OPP_V = 2 };
+
+struct Ring {
+    bool neg_tb, neg_lr;
+};
+
+inline Ring ring_of(int mode) {
+    return Ring{mode == BND_OPP_V, mode == BND_OPP_H};
+}
+
+// The linear relaxation a sweep applies (src/fluid_solver_cpu.cpp:107-108, :199-200).
+enum SweepKind : int { SWEEP_DIFFUSE = 0, SWEEP_PRESSURE = 1 };
+
+struct SweepArgs {
+    float* x;        // iterate (in place for Gauss-Seidel)
+    const float* b;  // right-hand side: x0 for diffuse, divergence for pressure
+    int R, P;
+    int kind;        // SweepKind
+    float a, denom;  // diffuse: a and 1 + 4a
+    Ring ring;
+    int K;           // total sweeps of this solve
+    int zero_start;  // iterate starts at 0 everywhere, ring included (pressure, :186)
+};
+
+// ---- launchers (stream-ordered, no synchronisation) ---------------------
+hipError_t launch_add_sources(hipStream_t s, float* x, float* x0, const float* src, int R, int P, float dt);
+hipError_t launch_max_speed(hipStream_t s, unsigned* out_bits, const float* u, const float* v, int R, int P);
+hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, const float* u,
+                                  const float* v, int R, int P, float dt0, const unsigned* max_bits);
+hipError_t launch_gather_velocity(hipStream_t s, float* u1, float* v1, const float* u0, const float* v0,
+                                  int R, int P, float dt0);
+hipError_t launch_divergence(hipStream_t s, float* d, const float* u, const float* v, int R, int P,
+                             float neg_half_h);
+hipError_t launch_gradient(hipStream_t s, float* u, float* v, const float* p, int R, int P, float h,
+                           int p_is_zero);
+hipError_t launch_set_boundary(hipStream_t s, float* x, int R, int P, Ring ring);
+hipError_t launch_fill(hipStream_t s, void* p, uint32_t value, size_t n_words);
+
+// Gauss-Seidel tile wavefront: one launch per level of the tile hyperplane.
+struct GsPlan {
+    int TW;                // tile columns
+    int GI, GJ;            // tile grid
+    int K;
+    int levels;
+    int2* work;            // device: (I << 16 | J, k) per tile-sweep, grouped by level
+    int* host_offsets;     // levels + 1 offsets into work
+};
+hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW);
+void gs_plan_destroy(GsPlan* plan);
+hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a);
+
+// Jacobi sweeps, `block` sweeps fused per launch; ping-pongs between x and
+// scratch and leaves the result in x.
+hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch, int block);
+int jacobi_launches(int K, int block);
+
+}  // namespace fs2d

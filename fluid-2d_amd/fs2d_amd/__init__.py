@@ -1,0 +1,208 @@
+"""Python binding of libfs2d_hip.so, the MI355X stable-fluids solver.
+
+A thin ctypes layer over the C ABI of ``include/fs2d_hip.h``; every call goes
+to the HIP library.  There is no CPU fallback: if the library is missing or
+has not been built, importing the solver raises.
+
+Arrays are numpy float32 (R, R), C-contiguous, ring included (the layout of
+the reference's ``grid<float>``, src/grid.hpp:35-44).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libfs2d_hip.so")
+
+GS_LEX, JACOBI = 0, 1
+DENSITY, VELOCITY_U, VELOCITY_V = 0, 1, 2
+
+_f, _i, _p = ctypes.c_float, ctypes.c_int, ctypes.c_void_p
+
+
+class Fs2dError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("diffuse_iterations", _i), ("pressure_iterations", _i), ("sweep_order", _i),
+                ("jacobi_block", _i), ("gs_tile_cols", _i), ("reserved", _i * 3)]
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load libfs2d_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Fs2dError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                        "(make -C fluid-2d_amd); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    H = _p
+    sig = {
+        "fs2d_default_params": (None, [ctypes.POINTER(Params)]),
+        "fs2d_hip_create": (_i, [_i, _i, ctypes.POINTER(Params), ctypes.POINTER(H)]),
+        "fs2d_hip_destroy": (_i, [H]),
+        "fs2d_hip_solve": (_i, [H, _p, _p, _f, _p, _p, _p, _p, _f, _f]),
+        "fs2d_hip_upload": (_i, [H, _p, _p, _p]),
+        "fs2d_hip_set_sources": (_i, [H, _p, _p, _p]),
+        "fs2d_hip_step_n": (_i, [H, _i, _f, _f, _f]),
+        "fs2d_hip_download": (_i, [H, _p, _p, _p]),
+        "fs2d_hip_sync": (_i, [H]),
+        "fs2d_hip_upload_device": (_i, [H, _p, _p, _p]),
+        "fs2d_hip_download_device": (_i, [H, _p, _p, _p]),
+        "fs2d_hip_stage_add_sources": (_i, [H, _i, _f]),
+        "fs2d_hip_stage_diffuse": (_i, [H, _i, _f, _f, _i]),
+        "fs2d_hip_stage_advect_density": (_i, [H, _f]),
+        "fs2d_hip_stage_advect_velocity": (_i, [H, _f]),
+        "fs2d_hip_stage_project": (_i, [H, _i]),
+        "fs2d_hip_stage_set_boundary": (_i, [H, _i]),
+        "fs2d_hip_download_pressure": (_i, [H, _p]),
+        "fs2d_hip_stream": (_p, [H]),
+        "fs2d_hip_time_pressure_sweeps": (_i, [H, _i, ctypes.POINTER(_f)]),
+        "fs2d_hip_launches_per_step": (_i, [H, ctypes.POINTER(_i)]),
+        "fs2d_hip_last_error": (ctypes.c_char_p, [H]),
+        "fs2d_hip_abi_version": (_i, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _arr(a: np.ndarray, R: int) -> int:
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or a.shape != (R, R) or not a.flags.c_contiguous:
+        raise ValueError(f"expected C-contiguous float32 array of shape ({R},{R})")
+    return a.ctypes.data
+
+
+class Solver:
+    """One simulation on the current HIP device (wraps an fs2d_handle).
+
+    The constructor mirrors ``fluid_solver_gpu(rows, cols)``
+    (src/fluid_solver_gpu.cuh:14); ``solve`` mirrors ``fluid_solver::solve``
+    (src/fluid_solver.hpp:16-24)."""
+
+    def __init__(self, rows: int, cols: int | None = None, diffuse_iterations: int = 20,
+                 pressure_iterations: int = 20, order: int = GS_LEX, jacobi_block: int = 0,
+                 gs_tile_cols: int = 0):
+        L = library()
+        cols = rows if cols is None else cols
+        prm = Params()
+        L.fs2d_default_params(ctypes.byref(prm))
+        prm.diffuse_iterations = diffuse_iterations
+        prm.pressure_iterations = pressure_iterations
+        prm.sweep_order = order
+        prm.jacobi_block = jacobi_block
+        prm.gs_tile_cols = gs_tile_cols
+        h = _p()
+        rc = L.fs2d_hip_create(rows, cols, ctypes.byref(prm), ctypes.byref(h))
+        if rc != 0:
+            raise Fs2dError(f"fs2d_hip_create failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
+        self._L, self._h, self.R = L, h, rows
+
+    def _ck(self, rc: int) -> None:
+        if rc != 0:
+            raise Fs2dError(f"fs2d error {rc}: {self._L.fs2d_hip_last_error(self._h).decode()}")
+
+    def close(self) -> None:
+        if self._h:
+            self._L.fs2d_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the reference interface ------------------------------------------------
+    def solve(self, rho, rho_src, diffusion, u, v, u_src, v_src, viscosity, dt) -> None:
+        R = self.R
+        self._ck(self._L.fs2d_hip_solve(self._h, _arr(rho, R), _arr(rho_src, R), diffusion, _arr(u, R),
+                                        _arr(v, R), _arr(u_src, R), _arr(v_src, R), viscosity, dt))
+
+    # -- resident API -------------------------------------------------------------
+    def upload(self, rho, u, v) -> None:
+        R = self.R
+        self._ck(self._L.fs2d_hip_upload(self._h, _arr(rho, R), _arr(u, R), _arr(v, R)))
+
+    def set_sources(self, rho_src=None, u_src=None, v_src=None) -> None:
+        R = self.R
+        ptr = lambda a: None if a is None else _arr(a, R)
+        self._ck(self._L.fs2d_hip_set_sources(self._h, ptr(rho_src), ptr(u_src), ptr(v_src)))
+
+    def step_n(self, n: int, diffusion: float, viscosity: float, dt: float) -> None:
+        self._ck(self._L.fs2d_hip_step_n(self._h, n, diffusion, viscosity, dt))
+
+    def sync(self) -> None:
+        self._ck(self._L.fs2d_hip_sync(self._h))
+
+    def download(self):
+        R = self.R
+        out = [np.empty((R, R), np.float32) for _ in range(3)]
+        self._ck(self._L.fs2d_hip_download(self._h, *(a.ctypes.data for a in out)))
+        return tuple(out)
+
+    def pressure(self) -> np.ndarray:
+        p = np.empty((self.R, self.R), np.float32)
+        self._ck(self._L.fs2d_hip_download_pressure(self._h, p.ctypes.data))
+        return p
+
+    # -- stages -------------------------------------------------------------------
+    def add_sources(self, field: int, dt: float) -> None:
+        self._ck(self._L.fs2d_hip_stage_add_sources(self._h, field, dt))
+
+    def diffuse(self, field: int, rate: float, dt: float, iterations: int) -> None:
+        self._ck(self._L.fs2d_hip_stage_diffuse(self._h, field, rate, dt, iterations))
+
+    def advect_density(self, dt: float) -> None:
+        self._ck(self._L.fs2d_hip_stage_advect_density(self._h, dt))
+
+    def advect_velocity(self, dt: float) -> None:
+        self._ck(self._L.fs2d_hip_stage_advect_velocity(self._h, dt))
+
+    def project(self, iterations: int) -> None:
+        self._ck(self._L.fs2d_hip_stage_project(self._h, iterations))
+
+    def set_boundary(self, field: int) -> None:
+        self._ck(self._L.fs2d_hip_stage_set_boundary(self._h, field))
+
+    # -- measurement --------------------------------------------------------------
+    def stream(self) -> int:
+        return int(self._L.fs2d_hip_stream(self._h) or 0)
+
+    def time_pressure_sweeps(self, reps: int) -> float:
+        ms = _f()
+        self._ck(self._L.fs2d_hip_time_pressure_sweeps(self._h, reps, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def launches_per_step(self) -> int:
+        n = _i()
+        self._ck(self._L.fs2d_hip_launches_per_step(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+
+EXPORTED_SYMBOLS = (
+    "fs2d_default_params", "fs2d_hip_create", "fs2d_hip_destroy", "fs2d_hip_solve", "fs2d_hip_upload",
+    "fs2d_hip_set_sources", "fs2d_hip_step_n", "fs2d_hip_download", "fs2d_hip_sync",
+    "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
+    "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
+    "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",
+    "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_launches_per_step",
+    "fs2d_hip_last_error", "fs2d_hip_abi_version",
+)

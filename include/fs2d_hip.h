@@ -1,0 +1,142 @@
+/*
+ * fs2d_hip.h -- C ABI of the MI355X stable-fluids solver (libfs2d_hip.so).
+ *
+ * Replaces the step of mworchel/fluid-2d behind its solver interface:
+ *   fluid_solver::solve          src/fluid_solver.hpp:16-24
+ *   fluid_solver_cpu::solve      src/fluid_solver_cpu.cpp:6-31   (the arithmetic it reproduces)
+ *   fluid_solver_gpu::solve      src/fluid_solver_gpu.cu:222-258 (the CUDA solver it sits beside)
+ *
+ * Plain C: no C++ or HIP types cross this boundary.  Every function returns an
+ * int status (FS2D_OK = 0); on failure fs2d_hip_last_error() describes it.
+ *
+ * Fields are rows x cols float32, row-major, index i*cols + j, including the
+ * one-cell boundary ring (src/grid.hpp:35-44).  Only square grids are
+ * accepted: the reference's grid::cols() returns rows (src/grid.hpp:20-22), so
+ * its solver is defined only for rows == cols.  rows >= 3.
+ *
+ * Threading: a handle owns one HIP stream and all of its device memory; use a
+ * handle from one host thread at a time (fluid_solver_gpu shares its
+ * temporaries between calls in the same way).
+ */
+#ifndef FS2D_HIP_H
+#define FS2D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS2D_ABI_VERSION 1
+
+enum fs2d_status {
+    FS2D_OK = 0,
+    FS2D_ERR_ARGUMENT = 1, /* invalid argument (sizes, NULL pointers, parameters) */
+    FS2D_ERR_HIP = 2,      /* a HIP runtime call failed (allocation, copy, launch) */
+    FS2D_ERR_STATE = 3     /* call not valid in the handle's current state */
+};
+
+/* Order of the diffuse and pressure relaxation sweeps. */
+enum fs2d_sweep_order {
+    /* In-place lexicographic Gauss-Seidel, the reference CPU solver's order
+     * (src/fluid_solver_cpu.cpp:104-113, :196-204); the iterates, and hence
+     * the step, equal the reference's bit for bit.  Default. */
+    FS2D_ORDER_GS_LEX = 0,
+    /* Jacobi (every neighbour from the previous iterate), temporally blocked
+     * in LDS.  A different relaxation than the reference's: it agrees with
+     * the reference only where the solve has converged. */
+    FS2D_ORDER_JACOBI = 1
+};
+
+/* Boundary rule of a field (src/fluid_solver_cpu.cpp:33-83). */
+enum fs2d_field {
+    FS2D_FIELD_DENSITY = 0,    /* continuous ring                    */
+    FS2D_FIELD_VELOCITY_U = 1, /* left/right edges negated (u)       */
+    FS2D_FIELD_VELOCITY_V = 2  /* top/bottom edges negated (v)       */
+};
+
+typedef struct fs2d_params {
+    int diffuse_iterations;  /* K_d; the reference's literal is 20 (src/fluid_solver_cpu.cpp:17,23-24) */
+    int pressure_iterations; /* K_p; the reference's literal is 20 (src/fluid_solver_cpu.cpp:25,30)    */
+    int sweep_order;         /* enum fs2d_sweep_order                                                   */
+    int jacobi_block;        /* Jacobi sweeps fused per launch (temporal block depth); 0 = autotuned    */
+    int gs_tile_cols;        /* Gauss-Seidel tile width in cells; 0 = default                            */
+    int reserved[3];         /* must be 0                                                                */
+} fs2d_params;
+
+typedef struct fs2d_handle fs2d_handle;
+
+/* Reference defaults: K_d = K_p = 20, Gauss-Seidel order. */
+void fs2d_default_params(fs2d_params* params);
+
+/* Allocate a solver for rows x cols grids on the current HIP device.
+ * Mirrors fluid_solver_gpu::fluid_solver_gpu(rows, cols)
+ * (src/fluid_solver_gpu.cuh:14, src/fluid_solver_gpu.cu:209-218).
+ * params may be NULL (reference defaults). */
+int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle** out);
+int fs2d_hip_destroy(fs2d_handle* h);
+
+/* One step with host arrays, synchronous: the semantics of
+ * fluid_solver::solve (src/fluid_solver.hpp:16-24).  density, u and v are
+ * read and updated in place, ring included; the three sources are read-only
+ * and only their interior is used (src/fluid_solver_cpu.cpp:88-90). */
+int fs2d_hip_solve(fs2d_handle* h, float* density, const float* density_source,
+                   float diffusion_rate, float* u, float* v, const float* u_source,
+                   const float* v_source, float viscosity, float dt);
+
+/* ---- resident API: state stays in HBM between steps -------------------- */
+/* Copy density, u, v (host, rows*cols floats each) into the handle. */
+int fs2d_hip_upload(fs2d_handle* h, const float* density, const float* u, const float* v);
+/* Set the sources applied by every following step; a NULL pointer means an
+ * all-zero source.  (simulation::update clears its sources after each call,
+ * src/simulation.cpp:62-64: set NULLs to reproduce that.) */
+int fs2d_hip_set_sources(fs2d_handle* h, const float* density_source, const float* u_source,
+                         const float* v_source);
+/* Enqueue n steps on the handle's stream; returns without waiting. */
+int fs2d_hip_step_n(fs2d_handle* h, int n, float diffusion_rate, float viscosity, float dt);
+/* Wait for the handle's stream, then copy the state out (any pointer may be NULL). */
+int fs2d_hip_download(fs2d_handle* h, float* density, float* u, float* v);
+int fs2d_hip_sync(fs2d_handle* h);
+
+/* ---- device-pointer variants (no host staging) --------------------------
+ * Arrays are rows*cols floats on the handle's device, row-major, unpadded.
+ * The copies are enqueued on the handle's stream. */
+int fs2d_hip_upload_device(fs2d_handle* h, const float* density, const float* u, const float* v);
+int fs2d_hip_download_device(fs2d_handle* h, float* density, float* u, float* v);
+
+/* ---- the reference's stages, on the resident state ----------------------
+ * Each runs one private stage of fluid_solver_cpu on the handle's fields, with
+ * the same boundary rule the step uses for that field.  For tests and
+ * tooling; fs2d_hip_step_n is the fused path. */
+/* src/fluid_solver_cpu.cpp:85-93: field += dt * source (interior). */
+int fs2d_hip_stage_add_sources(fs2d_handle* h, int field, float dt);
+/* src/fluid_solver_cpu.cpp:95-114 with `iterations` sweeps. */
+int fs2d_hip_stage_diffuse(fs2d_handle* h, int field, float rate, float dt, int iterations);
+/* src/fluid_solver_cpu.cpp:127-152: density scatter along (u, v). */
+int fs2d_hip_stage_advect_density(fs2d_handle* h, float dt);
+/* src/fluid_solver_cpu.cpp:26-29,153-174: u and v gathered along a copy of (u, v). */
+int fs2d_hip_stage_advect_velocity(fs2d_handle* h, float dt);
+/* src/fluid_solver_cpu.cpp:179-215 with `iterations` pressure sweeps. */
+int fs2d_hip_stage_project(fs2d_handle* h, int iterations);
+/* src/fluid_solver_cpu.cpp:33-83 on one field. */
+int fs2d_hip_stage_set_boundary(fs2d_handle* h, int field);
+/* Copy the last projection's pressure (rows*cols floats, ring included) to host. */
+int fs2d_hip_download_pressure(fs2d_handle* h, float* p);
+
+/* ---- measurement ------------------------------------------------------- */
+/* The handle's HIP stream (a hipStream_t), for callers that record events. */
+void* fs2d_hip_stream(fs2d_handle* h);
+/* Enqueue only the relaxation sweeps of one pressure solve (K_p sweeps on the
+ * current divergence) `reps` times and return the mean time of one solve in
+ * milliseconds, measured with HIP events on the handle's stream.  The state's
+ * velocity is not modified. */
+int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms_per_solve);
+/* Number of kernel launches one step enqueues (for per-launch averages). */
+int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches);
+
+/* Last error of the handle (or of the last failed create when h is NULL). */
+const char* fs2d_hip_last_error(const fs2d_handle* h);
+int fs2d_hip_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

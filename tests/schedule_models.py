@@ -1,0 +1,218 @@
+"""Host-side models of the device sweep schedules (fluid-2d_amd/csrc/fs2d_sweeps.hip).
+
+Each model replays the kernel's own decomposition -- tiles, launch levels,
+halo loads, the in-tile evaluation order, ring handling, buffer ping-pong --
+one cell at a time in float32 on the CPU, and asserts what the kernel relies
+on for correctness:
+
+* every read and write stays inside the R x R grid;
+* tile-sweeps launched together (one level of the Gauss-Seidel hyperplane)
+  never read a cell another of them writes, so they may run in any order;
+* within a Gauss-Seidel tile, lane r at step s consumes only values produced
+  at step s - 1 or loaded before the sweep.
+
+The tests compare the models with the oracle bit for bit, so a flaw in the
+schedule itself (a wrong level formula, a missing halo, a stale ring) shows
+up on the CPU before the kernels run on a GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+F = np.float32
+ROWS = 64  # Gauss-Seidel tile rows (one per lane)
+
+
+def _relax(kind, b, n, s, w, e, a, denom):
+    if kind == 0:
+        return F(F(b + F(a * F(F(F(n + s) + w) + e))) / denom)
+    return F(F(F(F(F(b + e) + w) + s) + n) / F(4.0))
+
+
+def _sgn(neg, x):
+    return F(-x) if neg else F(x)
+
+
+def gs_levels(R, K, TW):
+    GI = -(-(R - 2) // ROWS)
+    GJ = -(-(R - 2) // TW)
+    nlev = (GI - 1) + (GJ - 1) + 2 * (K - 1) + 1 if K > 0 else 0
+    levels = [[] for _ in range(nlev)]
+    for k in range(K):
+        for I in range(GI):
+            for J in range(GJ):
+                levels[I + J + 2 * k].append((I, J, k))
+    return levels
+
+
+def gs_tile_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, TW=64):
+    """In-place Gauss-Seidel solve by the tile-hyperplane schedule of k_gs_tile."""
+    R = x.shape[0]
+    a, denom = F(a), F(denom)
+    for level in gs_levels(R, K, TW):
+        snap = x.copy()
+        reads, writes, updates = [], [], []
+        for (I, J, k) in level:
+            rd, wr = set(), set()
+            gi0, gj0 = 1 + ROWS * I, 1 + TW * J
+            rows, cols = min(ROWS, R - 1 - gi0), min(TW, R - 1 - gj0)
+            first = k == 0
+            T = np.full((ROWS + 2, TW + 2), np.nan, np.float32)
+
+            def rd_(i, j):
+                assert 0 <= i < R and 0 <= j < R, (i, j)
+                rd.add((i, j))
+                return snap[i, j]
+
+            for r in range(-1, rows + 1):
+                gi = gi0 + r
+                ring_i = gi == 0 or gi == R - 1
+                for c in range(-1, cols + 1):
+                    if r in (-1, rows) and c in (-1, cols):
+                        continue
+                    gj = gj0 + c
+                    ring_j = gj == 0 or gj == R - 1
+                    if first and zero_start:
+                        val = rd_(gi, gj) if ((r == -1 and not ring_i) or (c == -1 and not ring_j)) else F(0)
+                    elif not first and ring_i:
+                        val = _sgn(neg_tb, rd_(1 if gi == 0 else R - 2, gj))
+                    elif not first and ring_j:
+                        val = _sgn(neg_lr, rd_(gi, 1 if gj == 0 else R - 2))
+                    else:
+                        val = rd_(gi, gj)
+                    T[r + 1, c + 1] = val
+            Bm = np.array([[b[gi0 + r, gj0 + c] for c in range(cols)] for r in range(rows)], np.float32).reshape(rows, cols)
+            # skewed sweep: step s, lane r relaxes column s - r
+            xprev = np.zeros(ROWS, np.float32)
+            produced = {}  # (r, c) -> step
+            for s in range(rows + cols - 1):
+                from_up = np.concatenate([[F(0)], xprev[:-1]])  # wave_shr:1
+                new_prev = xprev.copy()
+                for lane in range(ROWS):
+                    c = s - lane
+                    if not (lane < rows and 0 <= c < cols):
+                        continue
+                    if lane == 0:
+                        n = T[0, c + 1]
+                    else:
+                        assert produced.get((lane - 1, c)) == s - 1
+                        n = from_up[lane]
+                    if c == 0:
+                        w = T[lane + 1, 0]
+                    else:
+                        assert produced.get((lane, c - 1)) == s - 1
+                        w = xprev[lane]
+                    # south and east must still hold sweep k-1 values
+                    assert (lane + 1, c) not in produced and (lane, c + 1) not in produced
+                    so, e = T[lane + 2, c + 1], T[lane + 1, c + 2]
+                    assert not (np.isnan(so) or np.isnan(e) or np.isnan(n) or np.isnan(w))
+                    v = _relax(kind, Bm[lane, c], n, so, w, e, a, denom)
+                    T[lane + 1, c + 1] = v
+                    new_prev[lane] = v
+                    produced[(lane, c)] = s
+                xprev = new_prev
+            out = {}
+            for r in range(rows):
+                for c in range(cols):
+                    out[(gi0 + r, gj0 + c)] = T[r + 1, c + 1]
+            if k == K - 1:
+                top, bottom = gi0 == 1, gi0 + rows == R - 1
+                left, right = gj0 == 1, gj0 + cols == R - 1
+                for c in range(cols):
+                    if top:
+                        out[(0, gj0 + c)] = _sgn(neg_tb, T[1, c + 1])
+                    if bottom:
+                        out[(R - 1, gj0 + c)] = _sgn(neg_tb, T[rows, c + 1])
+                for r in range(rows):
+                    if left:
+                        out[(gi0 + r, 0)] = _sgn(neg_lr, T[r + 1, 1])
+                    if right:
+                        out[(gi0 + r, R - 1)] = _sgn(neg_lr, T[r + 1, cols])
+                corner = lambda v: F(F(0.5) * F(_sgn(neg_tb, v) + _sgn(neg_lr, v)))
+                if top and left:
+                    out[(0, 0)] = corner(T[1, 1])
+                if top and right:
+                    out[(0, R - 1)] = corner(T[1, cols])
+                if bottom and left:
+                    out[(R - 1, 0)] = corner(T[rows, 1])
+                if bottom and right:
+                    out[(R - 1, R - 1)] = corner(T[rows, cols])
+            for (i, j) in out:
+                assert 0 <= i < R and 0 <= j < R
+                wr.add((i, j))
+            reads.append(rd)
+            writes.append(wr)
+            updates.append(out)
+        # concurrency: no tile of this level reads or writes what another writes
+        for t in range(len(level)):
+            for u in range(len(level)):
+                if t != u:
+                    assert not (reads[t] & writes[u]), ("read/write overlap", level[t], level[u])
+                    assert not (writes[t] & writes[u]), ("write/write overlap", level[t], level[u])
+        for out in updates:
+            for (i, j), val in out.items():
+                x[i, j] = val
+    return x
+
+
+def jacobi_block_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, JX=64, JY=32):
+    """Jacobi solve by k_jacobi_block launches of M sweeps (ping-pong buffers).
+    Returns the result array (the kernel's x or scratch)."""
+    R = x.shape[0]
+    a, denom = F(a), F(denom)
+    src = x.copy()
+    zero = bool(zero_start)
+    done = 0
+    while done < K:
+        m = min(M, K - done)
+        dst = np.full((R, R), np.nan, np.float32)
+        W, H = JX + 2 * m, JY + 2 * m
+        for by in range(-(-(R - 2) // JY)):
+            for bx in range(-(-(R - 2) // JX)):
+                gi0, gj0 = 1 + JY * by - m, 1 + JX * bx - m
+                buf = [np.zeros((H, W), np.float32), np.full((H, W), np.nan, np.float32)]
+                valid = np.zeros((H, W), bool)
+                for r in range(H):
+                    for c in range(W):
+                        gi, gj = gi0 + r, gj0 + c
+                        if 0 <= gi < R and 0 <= gj < R:
+                            buf[0][r, c] = F(0) if zero else src[gi, gj]
+                            valid[r, c] = True
+                for s in range(1, m + 1):
+                    e = m - s
+                    cur, nxt = buf[(s - 1) & 1], buf[s & 1]
+                    newvalid = np.zeros((H, W), bool)
+                    for r in range(m - e, m + JY + e):
+                        for c in range(m - e, m + JX + e):
+                            gi, gj = gi0 + r, gj0 + c
+                            if not (1 <= gi <= R - 2 and 1 <= gj <= R - 2):
+                                continue
+                            assert 1 <= r <= H - 2 and 1 <= c <= W - 2
+                            for (rr, cc) in ((r - 1, c), (r + 1, c), (r, c - 1), (r, c + 1)):
+                                assert valid[rr, cc], ("stale read", s, gi, gj)
+                            v = _relax(kind, b[gi, gj], cur[r - 1, c], cur[r + 1, c], cur[r, c - 1], cur[r, c + 1], a, denom)
+                            tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                            cn = F(F(0.5) * F(tb + lr))
+                            tgt = []
+                            tgt.append((r, c, v))
+                            if gi == 1: tgt.append((r - 1, c, tb))
+                            if gi == R - 2: tgt.append((r + 1, c, tb))
+                            if gj == 1: tgt.append((r, c - 1, lr))
+                            if gj == R - 2: tgt.append((r, c + 1, lr))
+                            if gi == 1 and gj == 1: tgt.append((r - 1, c - 1, cn))
+                            if gi == 1 and gj == R - 2: tgt.append((r - 1, c + 1, cn))
+                            if gi == R - 2 and gj == 1: tgt.append((r + 1, c - 1, cn))
+                            if gi == R - 2 and gj == R - 2: tgt.append((r + 1, c + 1, cn))
+                            for (rr, cc, val) in tgt:
+                                if s < m:
+                                    assert 0 <= rr < H and 0 <= cc < W
+                                    nxt[rr, cc] = val
+                                    newvalid[rr, cc] = True
+                                else:
+                                    dst[gi0 + rr, gj0 + cc] = val
+                    valid = newvalid
+        assert not np.isnan(dst).any(), "unwritten cells after a Jacobi launch"
+        src = dst
+        zero = False
+        done += m
+    return src

@@ -1,0 +1,327 @@
+"""GPU parity of libfs2d_hip.so against the oracle and the reference's golden outputs.
+
+Every test drives the HIP solver through the C ABI (fs2d_amd -> ctypes ->
+libfs2d_hip.so) and compares with the CPU oracle on the same inputs, or with
+the fixtures the reference's own compiled solver wrote (tests/golden/).
+
+Tolerance: the Gauss-Seidel path and the Jacobi path (against the oracle's
+Jacobi order) must be BIT-IDENTICAL -- every stage evaluates the reference's
+float32 expressions in the reference's order with IEEE rounding, so any
+difference is a bug.  (rel-L2 and max|diff| are printed on failure.)
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import fs2d_amd
+import oracle
+from conftest import bits_equal, describe, golden
+from fs2d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GS, JAC = fs2d_amd.GS_LEX, fs2d_amd.JACOBI
+SIZES = [3, 4, 6, 10, 34, 67, 70, 130, 258]
+
+
+def assert_same(got, want, what):
+    assert bits_equal(got, want), f"{what}: {describe(got, want)}"
+
+
+def assert_same_nan(got, want, what):
+    gn, wn = np.isnan(got), np.isnan(want)
+    assert np.array_equal(gn, wn), f"{what}: NaN pattern differs"
+    g, w = np.where(gn, 0, got).astype(np.float32), np.where(wn, 0, want).astype(np.float32)
+    assert bits_equal(g, w), f"{what}: {describe(g, w)}"
+
+
+def make(R, state, order=GS, kd=20, kp=20, block=0):
+    rho, u, v, rs, us, vs = state
+    s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order, jacobi_block=block)
+    s.upload(rho, u, v)
+    s.set_sources(rs, us, vs)
+    return s
+
+
+def oracle_order(order):
+    return oracle.GS_LEX if order == GS else oracle.JACOBI
+
+
+# ---------------------------------------------------------------- stages
+@pytest.mark.parametrize("R", SIZES)
+@pytest.mark.parametrize("field", [0, 1, 2])
+def test_set_boundary(R, field, oracle_lib):
+    st = W.random_state(R, seed=R)
+    with make(R, st) as s:
+        s.set_boundary(field)
+        got = s.download()[field]
+    want = st[field].copy()
+    oracle_lib.set_boundary(want, field)
+    assert_same(got, want, f"set_boundary R={R} field={field}")
+
+
+@pytest.mark.parametrize("R", SIZES)
+def test_add_sources(R, oracle_lib):
+    st = W.random_state(R, seed=R)
+    with make(R, st) as s:
+        for f in range(3):
+            s.add_sources(f, 0.1)
+        got = s.download()
+    for f in range(3):
+        want = st[f].copy()
+        oracle_lib.add_sources(want, st[3 + f], 0.1)
+        assert_same(got[f], want, f"add_sources R={R} field={f}")
+
+
+@pytest.mark.parametrize("order,block", [(GS, 0), (JAC, 1), (JAC, 3), (JAC, 4), (JAC, 8)])
+@pytest.mark.parametrize("R", [4, 10, 67, 130, 258])
+@pytest.mark.parametrize("rate", [0.0, 1e-3, 5e-2])
+def test_diffuse(R, rate, order, block, oracle_lib):
+    st = W.random_state(R, seed=11 * R)
+    for K in (1, 2, 7, 20):
+        with make(R, st, order=order, block=block) as s:
+            for f in range(3):
+                s.diffuse(f, rate, 0.1, K)
+            got = s.download()
+        for f in range(3):
+            want = st[f].copy()
+            oracle_lib.diffuse(want, f, rate, 0.1, K, oracle_order(order))
+            assert_same(got[f], want, f"diffuse R={R} field={f} rate={rate} K={K} order={order} block={block}")
+
+
+@pytest.mark.parametrize("R", SIZES)
+@pytest.mark.parametrize("vel", [0.0, 0.05, 1.0, 6.0])
+def test_scatter_density(R, vel, oracle_lib):
+    st = W.random_state(R, seed=5 * R + 1, vel_scale=max(vel, 1e-9))
+    if vel == 0.0:
+        st = (st[0], np.zeros_like(st[1]), np.zeros_like(st[2])) + st[3:]
+    with make(R, st) as s:
+        s.advect_density(0.1)
+        got = s.download()[0]
+    want = st[0].copy()
+    oracle_lib.advect(want, st[1], st[2], 0, 0.1, True)
+    assert_same(got, want, f"scatter R={R} vel={vel}")
+
+
+@pytest.mark.parametrize("R", SIZES)
+@pytest.mark.parametrize("vel", [0.0, 0.05, 1.0, 6.0])
+def test_gather_velocity(R, vel, oracle_lib):
+    st = W.random_state(R, seed=3 * R + 2, vel_scale=max(vel, 1e-9))
+    if vel == 0.0:
+        st = (st[0], np.zeros_like(st[1]), np.zeros_like(st[2])) + st[3:]
+    with make(R, st) as s:
+        s.advect_velocity(0.1)
+        _, gu, gv = s.download()
+    tu, tv = st[1].copy(), st[2].copy()
+    wu, wv = st[1].copy(), st[2].copy()
+    oracle_lib.advect(wu, tu, tv, 1, 0.1, False)
+    oracle_lib.advect(wv, tu, tv, 2, 0.1, False)
+    assert_same(gu, wu, f"gather u R={R} vel={vel}")
+    assert_same(gv, wv, f"gather v R={R} vel={vel}")
+
+
+def test_non_finite_velocity(oracle_lib):
+    """NaN / inf velocities: scatter skips the source (the reference's size_t
+    conversion of NaN is undefined behaviour), the gather clamps NaN to R-1.5
+    through std::min/std::max exactly as the reference does."""
+    R = 34
+    st = list(W.random_state(R, seed=99, vel_scale=0.5))
+    st[1][5, 7] = np.nan
+    st[2][9, 3] = np.inf
+    st[1][20, 20] = -np.inf
+    st[2][12, 30] = np.nan
+    with make(R, st) as s:
+        s.advect_density(0.1)
+        s.advect_velocity(0.1)
+        got = s.download()
+    want = [x.copy() for x in st[:3]]
+    oracle_lib.advect(want[0], st[1], st[2], 0, 0.1, True)
+    tu, tv = want[1].copy(), want[2].copy()
+    oracle_lib.advect(want[1], tu, tv, 1, 0.1, False)
+    oracle_lib.advect(want[2], tu, tv, 2, 0.1, False)
+    for f in range(3):
+        assert_same_nan(got[f], want[f], f"non-finite field={f}")
+
+
+@pytest.mark.parametrize("order,block", [(GS, 0), (JAC, 2), (JAC, 4), (JAC, 5)])
+@pytest.mark.parametrize("R", [3, 4, 10, 67, 130, 258])
+def test_project(R, order, block, oracle_lib):
+    st = W.random_state(R, seed=7 * R, vel_scale=1.0)
+    for K in (0, 1, 3, 20):
+        with make(R, st, order=order, block=block) as s:
+            s.project(K)
+            _, gu, gv = s.download()
+            gp = s.pressure()
+        wu, wv = st[1].copy(), st[2].copy()
+        wp = oracle_lib.project(wu, wv, K, oracle_order(order), want_p=True)
+        assert_same(gp, wp, f"pressure R={R} K={K} order={order}")
+        assert_same(gu, wu, f"project u R={R} K={K} order={order}")
+        assert_same(gv, wv, f"project v R={R} K={K} order={order}")
+
+
+# ------------------------------------------------ the reference's own outputs
+@pytest.mark.parametrize("R", [4, 6, 10, 18, 34])
+def test_golden_stages(R):
+    """GPU stages against vectors written by the reference's compiled solver."""
+    g = golden(f"stages_R{R}.npz")
+    st = tuple(g[k] for k in ("rho", "u", "v", "rs", "us", "vs"))
+    for m in range(3):
+        # the fixture applies ring mode m to rho: load rho into field m
+        st_m = list(st)
+        st_m[m] = g["rho"]
+        with make(R, tuple(st_m)) as s:
+            s.set_boundary(m)
+            assert_same(s.download()[m], g[f"bnd{m}"], f"golden set_boundary mode={m}")
+    with make(R, st) as s:
+        s.add_sources(0, 0.1)
+        assert_same(s.download()[0], g["add"], "golden add_sources")
+    for m, key in ((0, "rho"), (1, "u"), (2, "v")):
+        for rate in (1e-3, 5e-2):
+            for it in (1, 7, 20):
+                with make(R, st) as s:
+                    s.diffuse(m, rate, 0.1, it)
+                    assert_same(s.download()[m], g[f"diff_m{m}_r{rate:g}_k{it}"], f"golden diffuse m={m} {rate} {it}")
+    for tag, (uk, vk) in (("slow", ("u", "v")), ("fast", ("uf", "vf"))):
+        stt = (g["rho"], g[uk], g[vk]) + st[3:]
+        with make(R, stt) as s:
+            s.advect_density(0.1)
+            assert_same(s.download()[0], g[f"scatter_{tag}"], f"golden scatter {tag}")
+        with make(R, stt) as s:
+            s.advect_velocity(0.1)
+            _, gu, gv = s.download()
+            assert_same(gu, g[f"gather_u_{tag}"], f"golden gather u {tag}")
+            assert_same(gv, g[f"gather_v_{tag}"], f"golden gather v {tag}")
+    for it in (1, 7, 20):
+        stt = (g["rho"], g["uf"], g["vf"]) + st[3:]
+        with make(R, stt) as s:
+            s.project(it)
+            _, gu, gv = s.download()
+            assert_same(gu, g[f"proj_u_k{it}"], f"golden project u K={it}")
+            assert_same(gv, g[f"proj_v_k{it}"], f"golden project v K={it}")
+
+
+@pytest.mark.parametrize("R", [4, 6, 10, 18, 34])
+def test_golden_solve(R):
+    """Whole steps (fluid_solver::solve) against the reference, 1 and 10 steps,
+    through the synchronous host-array entry point."""
+    g = golden(f"solve_R{R}.npz")
+    rho, u, v = g["rho"].copy(), g["u"].copy(), g["v"].copy()
+    with fs2d_amd.Solver(R) as s:
+        for step in range(1, 11):
+            s.solve(rho, g["rs"], 1e-3, u, v, g["us"], g["vs"], 1e-3, 0.1)
+            if step in (1, 10):
+                assert_same(rho, g[f"rho_s{step}"], f"golden solve rho R={R} step {step}")
+                assert_same(u, g[f"u_s{step}"], f"golden solve u R={R} step {step}")
+                assert_same(v, g[f"v_s{step}"], f"golden solve v R={R} step {step}")
+
+
+def test_config1_against_reference():
+    """Configuration 1: 128x128 + ring, zero start, disc sources every step,
+    dt=0.1, viscosity = diffusion = 0, K = 20, 200 steps; resident API."""
+    g = golden("config1_R130.npz")
+    R = 130
+    with fs2d_amd.Solver(R) as s:
+        s.upload(*W.zero_state(R))
+        s.set_sources(g["rs"], g["us"], g["vs"])
+        done = 0
+        for target in (1, 10, 200):
+            s.step_n(target - done, 0.0, 0.0, 0.1)
+            done = target
+            rho, u, v = s.download()
+            assert_same(rho, g[f"rho_s{target}"], f"config1 density step {target}")
+            assert_same(u, g[f"u_s{target}"], f"config1 u step {target}")
+            assert_same(v, g[f"v_s{target}"], f"config1 v step {target}")
+
+
+# ------------------------------------------------ whole steps vs the oracle
+@pytest.mark.parametrize("order,block", [(GS, 0), (JAC, 4)])
+@pytest.mark.parametrize("R", [67, 130, 258])
+def test_steps_vs_oracle(R, order, block, oracle_lib):
+    st = W.vortex_state(R, seed=R, n_vortices=16)
+    rs, us, vs = W.disc_sources(R, radius=R / 16)
+    rho, u, v = (x.copy() for x in st)
+    want = [x.copy() for x in st]
+    with fs2d_amd.Solver(R, diffuse_iterations=12, pressure_iterations=30, order=order, jacobi_block=block) as s:
+        s.upload(rho, u, v)
+        s.set_sources(rs, us, vs)
+        done = 0
+        for target in (1, 10):
+            s.step_n(target - done, 1e-4, 1e-5, 0.05)
+            for _ in range(target - done):
+                oracle_lib.solve(want[0], rs, 1e-4, want[1], want[2], us, vs, 1e-5, 0.05, 12, 30,
+                                 oracle_order(order))
+            done = target
+            got = s.download()
+            for f, name in enumerate(("density", "u", "v")):
+                assert_same(got[f], want[f], f"{name} R={R} order={order} after {target} steps")
+
+
+def test_config2_one_step(oracle_lib):
+    """Configuration 2 shape: 1024^2 interior, 64-vortex start, dt=0.05,
+    viscosity 1e-6, diffusion 1e-5, K = 40; one step, Gauss-Seidel order."""
+    R = 1026
+    st = W.vortex_state(R, seed=2)
+    zeros = np.zeros((R, R), np.float32)
+    got = [x.copy() for x in st]
+    with fs2d_amd.Solver(R, diffuse_iterations=40, pressure_iterations=40) as s:
+        s.solve(got[0], zeros, 1e-5, got[1], got[2], zeros, zeros, 1e-6, 0.05)
+    want = [x.copy() for x in st]
+    oracle_lib.solve(want[0], zeros, 1e-5, want[1], want[2], zeros, zeros, 1e-6, 0.05, 40, 40, oracle.GS_LEX)
+    for f, name in enumerate(("density", "u", "v")):
+        assert_same(got[f], want[f], f"config2 {name}")
+
+
+# ------------------------------------------------ full-size (4096^2) checks
+@pytest.fixture(scope="module")
+def big_state():
+    R = 4098
+    return W.vortex_state(R, seed=3)
+
+
+def test_full_size_stages(big_state, oracle_lib):
+    """4096^2 interior: each stage against the oracle with short solves (the
+    oracle cannot afford 80 sweeps here); exercises the full tile grid."""
+    R = 4098
+    rho, u, v = big_state
+    z = np.zeros((R, R), np.float32)
+    st = (rho, u, v, z, z, z)
+    with make(R, st, kd=3, kp=3) as s:
+        s.diffuse(0, 1e-5, 0.05, 3)
+        s.advect_density(0.05)
+        s.project(3)
+        got = s.download()
+    want = [rho.copy(), u.copy(), v.copy()]
+    oracle_lib.diffuse(want[0], 0, 1e-5, 0.05, 3)
+    oracle_lib.advect(want[0], want[1], want[2], 0, 0.05, True)
+    oracle_lib.project(want[1], want[2], 3)
+    for f in range(3):
+        assert_same(got[f], want[f], f"4096^2 stages field {f}")
+
+
+def test_full_size_step_deterministic(big_state):
+    """Two independent runs of 3 full steps (K = 80) are bit-identical, and the
+    second projection leaves the wide-stencil divergence below its input."""
+    R = 4098
+    rho, u, v = big_state
+    out = []
+    for _ in range(2):
+        with fs2d_amd.Solver(R, diffuse_iterations=80, pressure_iterations=80) as s:
+            s.upload(rho, u, v)
+            s.set_sources(None, None, None)
+            s.step_n(3, 1e-5, 1e-6, 0.05)
+            out.append(s.download())
+    for f in range(3):
+        assert bits_equal(out[0][f], out[1][f]), f"run-to-run difference in field {f}"
+    o = oracle.Oracle()
+    assert o.divergence_norm_f64(out[0][1], out[0][2]) < o.divergence_norm_f64(u, v)
+
+
+# ------------------------------------------------ errors
+def test_rejects_bad_shapes():
+    with pytest.raises(fs2d_amd.Fs2dError, match="square"):
+        fs2d_amd.Solver(10, 12)
+    with pytest.raises(fs2d_amd.Fs2dError):
+        fs2d_amd.Solver(2)
+    with pytest.raises(fs2d_amd.Fs2dError):
+        fs2d_amd.Solver(10, diffuse_iterations=-1)

@@ -1,0 +1,133 @@
+"""CPU-side tests: the C ABI library loads and exports what include/fs2d_hip.h
+declares, argument validation happens before any HIP call, the sweep
+schedules of the kernels reproduce the oracle (host-side models), and the
+workload generator is deterministic.  No GPU compute is issued here."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fs2d_amd
+import oracle
+from conftest import ROOT, bits_equal
+from fs2d_amd import workloads as W
+from schedule_models import gs_tile_model, jacobi_block_model
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "fs2d_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fs2d_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = fs2d_amd.library()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(syms) == set(fs2d_amd.EXPORTED_SYMBOLS)
+    assert lib.fs2d_hip_abi_version() == 1
+
+
+def test_default_params_are_the_reference_literals():
+    lib = fs2d_amd.library()
+    p = fs2d_amd.Params()
+    lib.fs2d_default_params(ctypes.byref(p))
+    assert (p.diffuse_iterations, p.pressure_iterations, p.sweep_order) == (20, 20, fs2d_amd.GS_LEX)
+
+
+@pytest.mark.parametrize("rows,cols,kw,msg", [
+    (10, 12, {}, "square"),
+    (2, 2, {}, "rows"),
+    (10, 10, {"diffuse_iterations": -1}, "iteration"),
+    (10, 10, {"order": 7}, "sweep_order"),
+    (10, 10, {"jacobi_block": 9}, "jacobi_block"),
+    (10, 10, {"gs_tile_cols": 48}, "gs_tile_cols"),
+])
+def test_create_validates_arguments_before_touching_the_device(rows, cols, kw, msg):
+    with pytest.raises(fs2d_amd.Fs2dError, match=msg):
+        fs2d_amd.Solver(rows, cols, **kw)
+
+
+def test_oracle_is_not_linked_into_the_product():
+    """The product library must not depend on the oracle (no CPU fallback)."""
+    so = open(fs2d_amd.LIB_PATH, "rb").read()
+    assert b"fso_" not in so and b"libfs2d_oracle" not in so and b"ref_solve" not in so
+
+
+# ------------------------------------------------ schedule models
+@pytest.mark.parametrize("R,TW", [(3, 64), (10, 64), (67, 64), (70, 128)])
+def test_gs_tile_schedule_reproduces_gauss_seidel(R, TW, oracle_lib):
+    rho, u, v, *_ = W.random_state(R, seed=R)
+    for mode in range(3):
+        for K in (1, 3):
+            a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+            den = np.float32(1) + np.float32(4) * a
+            x = rho.copy()
+            gs_tile_model(x, rho.copy(), K, 0, a, den, mode == 2, mode == 1, 0, TW)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert bits_equal(x, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    a, b = u.copy(), v.copy()
+    p_ref = oracle_lib.project(a, b, 3, oracle.GS_LEX, want_p=True)
+    p = np.full((R, R), 7.0, np.float32)  # garbage must never be read at sweep 0
+    gs_tile_model(p, d, 3, 1, 0, 0, False, False, 1, TW)
+    assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R", [3, 10, 67])
+@pytest.mark.parametrize("M", [1, 2, 3])
+def test_jacobi_block_schedule_reproduces_jacobi(R, M, oracle_lib):
+    rho, u, v, *_ = W.random_state(R, seed=R + 1)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        y = jacobi_block_model(rho.copy(), rho.copy(), 5, 0, a, den, mode == 2, mode == 1, 0, M)
+        ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, 5, oracle.JACOBI)
+        assert bits_equal(y, ref), mode
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 4, oracle.JACOBI, want_p=True)
+    p = jacobi_block_model(np.full((R, R), 7.0, np.float32), d, 4, 1, 0, 0, False, False, 1, M)
+    assert bits_equal(p, p_ref)
+
+
+# ------------------------------------------------ workloads
+def test_splitmix64_known_answer():
+    # first outputs of SplitMix64 seeded with 0 (Vigna's reference implementation)
+    g = W.SplitMix64(0)
+    assert [g.next_u64() for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    g1, g2 = W.SplitMix64(42), W.SplitMix64(42)
+    arr = g1.uniform_array(5)
+    assert [g2.uniform() for _ in range(5)] == list(arr)
+    assert g1.next_u64() == g2.next_u64()
+
+
+def test_config1_sources():
+    rs, us, vs = W.disc_sources(130)
+    disc = rs > 0
+    assert int(disc.sum()) == 208  # cells with (i-64.5)^2 + (j-64.5)^2 <= 64
+    assert np.all(vs[disc] == -5.0) and np.all(rs[disc] == 100.0) and not us.any()
+
+
+def test_vortex_state_deterministic_and_divergence_free():
+    a = W.vortex_state(66, seed=5, n_vortices=8)
+    b = W.vortex_state(66, seed=5, n_vortices=8)
+    for x, y in zip(a, b):
+        assert bits_equal(x, y)
+    rho, u, v = a
+    speed = np.sqrt(u[1:-1, 1:-1].astype(np.float64) ** 2 + v[1:-1, 1:-1] ** 2).max()
+    assert abs(speed - 2.0 / 66) < 1e-6
+    # resolved vortices (radii 10-40 cells): the discrete divergence of the
+    # analytic curl field is a small fraction of its derivative terms
+    uu, vv = W.vortex_velocity(130, 8, W.SplitMix64(3), radius_scale=1.0)
+    uu, vv = uu.astype(np.float64), vv.astype(np.float64)
+    du, dv = uu[1:-1, 2:] - uu[1:-1, :-2], vv[2:, 1:-1] - vv[:-2, 1:-1]
+    inner = (slice(3, -3), slice(3, -3))
+    assert np.abs(du + dv)[inner].max() < 0.02 * (np.abs(du) + np.abs(dv))[inner].max()
+    assert 0.0 <= rho.min() and rho.max() <= 1.0
