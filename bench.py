@@ -1,0 +1,171 @@
+"""Benchmark of the MI355X stable-fluids step (one GPU).
+
+Workload (the bandwidth-roofline configuration): 4096 x 4096 interior cells
+plus the boundary ring (R = 4098), float32, dt = 0.05, viscosity 1e-6,
+diffusion 1e-5, 80 relaxation sweeps for each diffuse and pressure solve,
+seeded 64-vortex start with smoothed-noise density, no sources.  A "step" is
+one fluid_solver::solve (src/fluid_solver_cpu.cpp:6-31) on the resident
+state; inputs are in HBM before the timed region starts.
+
+The headline is the Gauss-Seidel order, whose results equal the reference
+solver's bit for bit.  The Jacobi order (a different relaxation, agreeing with
+the reference only where the solves converge) is reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096]
+                    [--order gs|jacobi] [--no-cpu] [--no-jacobi]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "fluid-2d_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def bytes_per_cell_step(kd: int, kp: int, a_dens: bool, a_visc: bool) -> int:
+    """Algorithmic HBM bytes per interior cell per step: every stage reads and
+    writes each distinct field once (float32).  add rho 12, diffuse rho 12*Kd,
+    scatter 16, add u,v 24, diffuse u,v 24*Kd, two projections of
+    (divergence 12 + 12*Kp + gradient 20), velocity gather 16.  A diffuse with
+    a = 0 earns no credit."""
+    b = 12 + 16 + 24 + 2 * (32 + 12 * kp) + 16
+    if a_dens:
+        b += 12 * kd
+    if a_visc:
+        b += 24 * kd
+    return b
+
+
+def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state, params):
+    import fs2d_amd
+
+    R = n + 2
+    diff, visc, dt = params
+    s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order)
+    s.upload(*state)
+    s.set_sources(None, None, None)
+    s.step_n(warmup, diff, visc, dt)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step_n(steps, diff, visc, dt)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    # dominant kernel: the pressure relaxation (K_p sweeps), HIP events on the
+    # solver's own stream around repeated solves
+    reps = max(3, min(20, int(0.5 / max(elapsed / steps / 5, 1e-4))))
+    ms_solve = s.time_pressure_sweeps(reps)
+    launches = s.launches_per_step()
+    s.close()
+    return elapsed / steps, ms_solve, launches
+
+
+def cpu_baseline(n: int, kd: int, kp: int, state, params, budget_s: float):
+    """The oracle (a bit-exact restatement of the reference CPU solver,
+    single-threaded like the reference) on one step of the same workload."""
+    import numpy as np
+
+    import oracle
+
+    O = oracle.Oracle()
+    R = n + 2
+    diff, visc, dt = params
+    rho, u, v = (x.copy() for x in state)
+    z = np.zeros((R, R), np.float32)
+    t0 = time.perf_counter()
+    O.solve(rho, z, diff, u, v, z, z, visc, dt, kd, kp, oracle.GS_LEX)
+    sec = time.perf_counter() - t0
+    return {"value": round(1.0 / sec, 6), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 full step at {n}x{n}+ring, K_d=K_p={kd}, Gauss-Seidel order, "
+                      f"oracle/fs2d_oracle.c (bit-identical to the reference CPU solver), "
+                      f"one host thread, {sec:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=4096, help="interior cells per side")
+    ap.add_argument("--iters", type=int, default=80, help="K_d = K_p")
+    ap.add_argument("--order", choices=("gs", "jacobi"), default="gs")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-jacobi", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=120.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(a single grid would shard as row strips with halo exchange, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from fs2d_amd import workloads as W
+
+    n, K = args.n, args.iters
+    R = n + 2
+    params = (1e-5, 1e-6, 0.05)  # diffusion, viscosity, dt
+    state = W.vortex_state(R, seed=3)
+    order = 0 if args.order == "gs" else 1
+
+    sec_step, ms_solve, launches = run_gpu(order, n, K, K, args.steps, args.warmup, state, params)
+    cells = n * n
+    bpc = bytes_per_cell_step(K, K, params[0] != 0, params[1] != 0)
+    step_gbs = bpc * cells / sec_step / 1e9
+    solve_bytes = 12 * cells * K  # read p, read d, write p per cell per sweep
+    solve_gbs = solve_bytes / (ms_solve * 1e-3) / 1e9
+    out = {
+        "metric": "steps/s (plain stable-fluids step, 4096^2 interior, K=80)",
+        "value": round(1.0 / sec_step, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_step * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 64-vortex velocity, smoothed-noise density)",
+        "config": {"workload": f"stable-fluids step {n}x{n}+ring, dt=0.05, visc=1e-6, diff=1e-5, "
+                               f"K_d=K_p={K}, {'Gauss-Seidel (reference order)' if order == 0 else 'Jacobi'}",
+                   "order": args.order, "n": n, "iterations": K},
+        "cell_updates_per_s": round(cells / sec_step, 1),
+        "step_hbm_gbs_algorithmic": round(step_gbs, 1),
+        "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "bytes_per_cell_step": bpc,
+        "launches_per_step": launches,
+        "roofline": {
+            "kernel": "k_gs_tile<64,pressure>" if order == 0 else "k_jacobi_block",
+            "bound": "hbm",
+            "achieved": round(solve_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(solve_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "unit_of_work": f"one pressure solve: {K} sweeps x {cells} cells x 12 B, {ms_solve:.3f} ms",
+        },
+    }
+    if order == 0 and not args.no_jacobi:
+        js, jms, jl = run_gpu(1, n, K, K, args.steps, args.warmup, state, params)
+        jg = bpc * cells / js / 1e9
+        out["jacobi"] = {"value": round(1.0 / js, 3), "ms_per_step": round(js * 1e3, 3),
+                         "step_hbm_gbs_algorithmic": round(jg, 1), "step_roofline_frac": round(jg / HBM_PEAK_GBS, 4),
+                         "pressure_solve_ms": round(jms, 4), "launches_per_step": jl}
+    if not args.no_cpu:
+        cb = cpu_baseline(n, K, K, state, params, args.cpu_budget)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(out["value"] / cb["value"], 1)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

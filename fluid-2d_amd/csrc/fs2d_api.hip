@@ -37,6 +37,7 @@ struct fs2d_handle {
     unsigned* max_bits = nullptr;
     std::map<int, GsPlan> gs_plans;  // by sweep count
     int gs_tw = 64;
+    int gs_block = 8;
     int jacobi_block = 4;
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
     std::string err;
@@ -80,7 +81,8 @@ hipError_t gs_plan_for(fs2d_handle* h, int K, const GsPlan** out) {
     auto it = h->gs_plans.find(K);
     if (it == h->gs_plans.end()) {
         GsPlan plan{};
-        hipError_t e = gs_plan_create(&plan, h->R, K, h->gs_tw);
+        hipError_t e = (h->gs_block > 1) ? gs_unit_plan_create(&plan, h->R, K, h->gs_tw, h->gs_block)
+                                         : gs_plan_create(&plan, h->R, K, h->gs_tw);
         if (e != hipSuccess) {
             gs_plan_destroy(&plan);
             return e;
@@ -213,6 +215,10 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: jacobi_block must be in [0, 8]");
     if (prm.gs_tile_cols != 0 && prm.gs_tile_cols != 64 && prm.gs_tile_cols != 128)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_tile_cols must be 0, 64 or 128");
+    if (prm.gs_block != 0 && prm.gs_block != 1 && prm.gs_block != 2 && prm.gs_block != 4 && prm.gs_block != 8)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_block must be 0, 1, 2, 4 or 8");
+    if (prm.gs_block > 1 && prm.gs_tile_cols == 128)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_block > 1 needs gs_tile_cols 64");
     for (int r : prm.reserved)
         if (r != 0) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: reserved fields must be 0");
 
@@ -222,6 +228,7 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     h->field_bytes = sizeof(float) * static_cast<size_t>(h->R) * h->P;
     h->prm = prm;
     h->gs_tw = prm.gs_tile_cols ? prm.gs_tile_cols : 64;
+    h->gs_block = prm.gs_block ? prm.gs_block : 8;
     h->jacobi_block = prm.jacobi_block ? prm.jacobi_block : 4;
     auto bail = [&](hipError_t e, const char* what) {
         std::string msg = std::string("fs2d_hip_create: ") + what + ": " + hipGetErrorString(e);

@@ -46,93 +46,312 @@ __device__ __forceinline__ float relax(int kind, float b, float n, float s, floa
 // ===================== Gauss-Seidel tile hyperplane ======================
 constexpr int GS_ROWS = 64;  // tile rows = wave lanes
 
-template <int TW>
+// Value of iterate cell (gi, gj) as sweep k sees it (gi, gj already clamped
+// into the grid): sweep 0 reads the stored ring, later sweeps the
+// sign-adjusted adjacent interior cell of sweep k-1.
+__device__ __forceinline__ const float* ring_source(const float* x, int gi, int gj, int R, int P, bool virtual_ring,
+                                                    Ring rg, bool& neg, bool& ring_i, bool& ring_j) {
+    ring_i = (gi == 0) | (gi == R - 1);
+    ring_j = (gj == 0) | (gj == R - 1);
+    int si = gi, sj = gj;
+    neg = false;
+    if (virtual_ring) {
+        if (ring_i) {
+            si = (gi == 0) ? 1 : R - 2;
+            neg = rg.neg_tb;
+        } else if (ring_j) {
+            sj = (gj == 0) ? 1 : R - 2;
+            neg = rg.neg_lr;
+        }
+    }
+    return x + (size_t)si * P + sj;
+}
+
+template <int KIND>
+__device__ __forceinline__ float relax_k(float b, float n, float s, float w, float e, float a, float denom) {
+    if (KIND == SWEEP_DIFFUSE) return (b + a * (((n + s) + w) + e)) / denom;
+    return ((((b + e) + w) + s) + n) / 4.0f;
+}
+
+template <int TW, int KIND>
 __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restrict__ work) {
-    // T: rows -1..64, cols -1..TW of the tile (row pitch TW + 2 is even, so the
-    // skewed access (r, s - r) hits 64 distinct banks).  Bm: right-hand side.
-    __shared__ float T[GS_ROWS + 2][TW + 2];
-    __shared__ float Bm[GS_ROWS][TW];
+    // T holds tile rows -1..64 and columns -1..TW (row pitch TP = TW + 2 is
+    // even, so the skewed access (r, s - r) of 64 lanes hits 64 distinct
+    // banks); Bm holds the right-hand side of the tile.
+    constexpr int TP = TW + 2;
+    constexpr int NQ = (TW + 2 + 63) / 64;  // halo columns per lane
+    constexpr int NB = TW / 64;             // right-hand-side columns per lane
+    constexpr int CH = 22;                  // rows per batch of in-flight loads (66 = 3 * 22)
+    __shared__ float T[(GS_ROWS + 2) * TP];
+    __shared__ float Bm[GS_ROWS * TW];
 
     const int2 wk = work[blockIdx.x];
     const int I = wk.x >> 16, J = wk.x & 0xffff, k = wk.y;
     const int R = A.R, P = A.P;
     float* __restrict__ x = A.x;
+    const float* __restrict__ b = A.b;
     const int gi0 = 1 + GS_ROWS * I, gj0 = 1 + TW * J;
     const int rows = min(GS_ROWS, R - 1 - gi0), cols = min(TW, R - 1 - gj0);
     const int lane = threadIdx.x;
     const bool first = (k == 0);
     const bool zero_first = first && A.zero_start;
+    const Ring rg = A.ring;
 
-    // ---- load the sweep k-1 iterate with its halo (corners of the halo are
-    // never read and are skipped).
-    for (int r = -1; r <= rows; ++r) {
-        const int gi = gi0 + r;
-        const bool ring_i = (gi == 0 || gi == R - 1);
-        for (int c = lane - 1; c <= cols; c += 64) {
-            if ((r == -1 || r == rows) && (c == -1 || c == cols)) continue;
-            const int gj = gj0 + c;
-            const bool ring_j = (gj == 0 || gj == R - 1);
-            float val;
-            if (zero_first) {
-                // Iterate starts at 0 (ring included); only the north and west
-                // halos already hold sweep 0 of the neighbouring tiles.
-                val = ((r == -1 && !ring_i) || (c == -1 && !ring_j)) ? x[(size_t)gi * P + gj] : 0.f;
-            } else if (!first && ring_i) {
-                val = sgn(A.ring.neg_tb, x[(size_t)(gi == 0 ? 1 : R - 2) * P + gj]);
-            } else if (!first && ring_j) {
-                val = sgn(A.ring.neg_lr, x[(size_t)gi * P + (gj == 0 ? 1 : R - 2)]);
-            } else {
-                val = x[(size_t)gi * P + gj];
+    // ---- load the sweep k-1 iterate with its halo.  A batch of loads is in
+    // flight before any is consumed.  Tiles whose halo stays off the ring use
+    // plain addressing; the others select each source address branch-free
+    // (clamped into the grid; cells outside the tile and the halo corners
+    // receive values that are never read).
+    const bool plain = !zero_first && gi0 >= 2 && gi0 + GS_ROWS <= R - 2 && gj0 >= 2 && gj0 + TW <= R - 2;
+    if (plain) {
+        const float* base = x + (size_t)(gi0 - 1) * P + (gj0 - 1) + lane;
+#pragma unroll
+        for (int r0 = 0; r0 < GS_ROWS + 2; r0 += CH) {
+            float vals[CH][NQ];
+#pragma unroll
+            for (int q = 0; q < CH; ++q)
+#pragma unroll
+                for (int t = 0; t < NQ; ++t)
+                    vals[q][t] = (lane + 64 * t <= TW + 1) ? base[(size_t)(r0 + q) * P + 64 * t] : 0.f;
+#pragma unroll
+            for (int q = 0; q < CH; ++q)
+#pragma unroll
+                for (int t = 0; t < NQ; ++t)
+                    if (lane + 64 * t <= TW + 1) T[(r0 + q) * TP + lane + 64 * t] = vals[q][t];
+        }
+    } else {
+#pragma unroll
+        for (int r0 = -1; r0 <= GS_ROWS; r0 += CH) {
+            float vals[CH][NQ];
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+#pragma unroll
+                for (int t = 0; t < NQ; ++t) {
+                    const int r = r0 + q, c = lane - 1 + 64 * t;
+                    const int gi = min(gi0 + r, R - 1), gj = min(gj0 + c, R - 1);
+                    bool neg, ring_i, ring_j;
+                    const float* src = ring_source(x, gi, gj, R, P, !first, rg, neg, ring_i, ring_j);
+                    float v = *src;
+                    v = neg ? -v : v;
+                    // iterate starting at 0: only the north / west halos
+                    // already hold sweep 0 of the neighbouring tiles
+                    if (zero_first && !((r == -1 && !ring_i) || (c == -1 && !ring_j))) v = 0.f;
+                    vals[q][t] = v;
+                }
             }
-            T[r + 1][c + 1] = val;
+#pragma unroll
+            for (int q = 0; q < CH; ++q)
+#pragma unroll
+                for (int t = 0; t < NQ; ++t) {
+                    const int r = r0 + q, c = lane - 1 + 64 * t;
+                    if (c <= TW) T[(r + 1) * TP + c + 1] = vals[q][t];
+                }
         }
     }
-    for (int r = 0; r < rows; ++r)
-        for (int c = lane; c < cols; c += 64) Bm[r][c] = A.b[(size_t)(gi0 + r) * P + gj0 + c];
+#pragma unroll
+    for (int r0 = 0; r0 < GS_ROWS; r0 += 32) {
+        float vals[32][NB];
+#pragma unroll
+        for (int q = 0; q < 32; ++q)
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const int r = min(r0 + q, rows - 1), c = min(lane + 64 * t, cols - 1);
+                vals[q][t] = b[(size_t)(gi0 + r) * P + gj0 + c];
+            }
+#pragma unroll
+        for (int q = 0; q < 32; ++q)
+#pragma unroll
+            for (int t = 0; t < NB; ++t) Bm[(r0 + q) * TW + lane + 64 * t] = vals[q][t];
+    }
     __syncthreads();
 
-    // ---- skewed sweep: at step s lane r relaxes column s - r.
-    const int kind = A.kind;
+    // ---- skewed sweep: at step s lane r relaxes column s - r.  The LDS
+    // operands of step s + 1 are read during step s (they are old values:
+    // lane r + 1 overwrites cell (r + 1, c + 1) only at step s + 2), so only
+    // xprev and the DPP shift carry the dependency from step to step.
+    // Inactive lanes compute on clamped operands and discard the result.
     const float a = A.a, denom = A.denom;
-    float xprev = 0.f;
     const int steps = rows + cols - 1;
+    const bool lane_ok = lane < rows;
+    const float w_halo = T[(lane + 1) * TP];
+    float* const trow = T + (lane + 1) * TP;
+    float xprev = 0.f;
+    int cc = min(max(-lane, 0), TW - 1);
+    float so_n = trow[TP + cc + 1], e_n = trow[cc + 2], b_n = Bm[lane * TW + cc], nh_n = T[cc + 1];
     for (int s = 0; s < steps; ++s) {
         const int c = s - lane;
+        const bool active = lane_ok && c >= 0 && c < cols;
+        const int cw = cc;
+        const float so = so_n, e = e_n, bb = b_n, n_halo = nh_n;
+        cc = min(max(c + 1, 0), TW - 1);
+        so_n = trow[TP + cc + 1];
+        e_n = trow[cc + 2];
+        b_n = Bm[lane * TW + cc];
+        nh_n = T[cc + 1];
         // previous result of lane - 1 = cell (lane - 1, c) of this sweep
         const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-        if (lane < rows && c >= 0 && c < cols) {
-            const float n = (lane == 0) ? T[0][c + 1] : from_up;
-            const float w = (c == 0) ? T[lane + 1][0] : xprev;
-            const float so = T[lane + 2][c + 1];
-            const float e = T[lane + 1][c + 2];
-            const float v = relax(kind, Bm[lane][c], n, so, w, e, a, denom);
-            T[lane + 1][c + 1] = v;
-            xprev = v;
-        }
+        const float n = (lane == 0) ? n_halo : from_up;
+        const float w = (c == 0) ? w_halo : xprev;
+        const float v = relax_k<KIND>(bb, n, so, w, e, a, denom);
+        if (active) trow[cw + 1] = v;
+        xprev = active ? v : xprev;
     }
     __syncthreads();
 
     // ---- store the tile; the last sweep also writes the ring it borders.
-    for (int r = 0; r < rows; ++r)
-        for (int c = lane; c < cols; c += 64) x[(size_t)(gi0 + r) * P + gj0 + c] = T[r + 1][c + 1];
+#pragma unroll 8
+    for (int r = 0; r < GS_ROWS; ++r)
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            const int c = lane + 64 * t;
+            if (r < rows && c < cols) x[(size_t)(gi0 + r) * P + gj0 + c] = T[(r + 1) * TP + c + 1];
+        }
     if (k == A.K - 1) {
         const bool top = (gi0 == 1), bottom = (gi0 + rows == R - 1);
         const bool left = (gj0 == 1), right = (gj0 + cols == R - 1);
-        const Ring rg = A.ring;
-        if (top)
-            for (int c = lane; c < cols; c += 64) x[gj0 + c] = sgn(rg.neg_tb, T[1][c + 1]);
-        if (bottom)
-            for (int c = lane; c < cols; c += 64) x[(size_t)(R - 1) * P + gj0 + c] = sgn(rg.neg_tb, T[rows][c + 1]);
-        if (left)
-            for (int r = lane; r < rows; r += 64) x[(size_t)(gi0 + r) * P] = sgn(rg.neg_lr, T[r + 1][1]);
-        if (right)
-            for (int r = lane; r < rows; r += 64) x[(size_t)(gi0 + r) * P + R - 1] = sgn(rg.neg_lr, T[r + 1][cols]);
+        for (int c = lane; c < cols; c += 64) {
+            if (top) x[gj0 + c] = sgn(rg.neg_tb, T[TP + c + 1]);
+            if (bottom) x[(size_t)(R - 1) * P + gj0 + c] = sgn(rg.neg_tb, T[rows * TP + c + 1]);
+        }
+        if (lane < rows) {
+            if (left) x[(size_t)(gi0 + lane) * P] = sgn(rg.neg_lr, T[(lane + 1) * TP + 1]);
+            if (right) x[(size_t)(gi0 + lane) * P + R - 1] = sgn(rg.neg_lr, T[(lane + 1) * TP + cols]);
+        }
         if (lane == 0) {
             auto corner = [&](float v) { return 0.5f * (sgn(rg.neg_tb, v) + sgn(rg.neg_lr, v)); };
-            if (top && left) x[0] = corner(T[1][1]);
-            if (top && right) x[R - 1] = corner(T[1][cols]);
-            if (bottom && left) x[(size_t)(R - 1) * P] = corner(T[rows][1]);
-            if (bottom && right) x[(size_t)(R - 1) * P + R - 1] = corner(T[rows][cols]);
+            if (top && left) x[0] = corner(T[TP + 1]);
+            if (top && right) x[R - 1] = corner(T[TP + cols]);
+            if (bottom && left) x[(size_t)(R - 1) * P] = corner(T[rows * TP + 1]);
+            if (bottom && right) x[(size_t)(R - 1) * P + R - 1] = corner(T[rows * TP + cols]);
+        }
+    }
+}
+
+// ============ Gauss-Seidel, temporally blocked (M sweeps per unit) ==========
+// In skewed coordinates i' = i + k, j' = j + k every Gauss-Seidel dependence
+// points to smaller-or-equal (i', j', k), so rectangular tiles of
+// 64 x TW cells x M sweeps are legal.  A unit (I', J', kappa) covers
+// i' in 64 I' + [1, 64], j' in TW J' + [1, TW], k in kappa M + [0, M).  Wave w
+// relaxes sweep kappa M + w; lane r owns skewed row i' = 64 I' + 1 + r and at
+// step s relaxes skewed column j' = TW J' + 1 + s - r.  In these coordinates all
+// M sweeps of a cell's neighbourhood sit on the same anti-diagonal: a cell's
+// sweep-k value is written at step t, read by sweep k + 1 at step t + 1 and
+// overwritten at step t + 2, so the M waves relax in lockstep, in place in LDS,
+// with one barrier per step.  Units with equal I' + J' + kappa touch disjoint
+// cells and are launched together.
+template <int TW, int M, int KIND>
+__global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __restrict__ work) {
+    constexpr int NT = 64 * M;
+    constexpr int XH = GS_ROWS + 1 + M, XW = TW + 1 + M;            // iterate region
+    constexpr int XP = XW + (XW & 1);                                // even pitch: skew is conflict-free
+    constexpr int BH = GS_ROWS - 1 + M, BW = TW - 1 + M;             // right-hand side region
+    constexpr int BP = BW + (BW & 1);
+    constexpr int NX = (XH * XW + NT - 1) / NT, NBX = (BH * BW + NT - 1) / NT;
+    __shared__ float X[XH * XP];
+    __shared__ float Bs[BH * BP];
+
+    const int2 wk = work[blockIdx.x];
+    const int I = wk.x >> 16, J = wk.x & 0xffff, kap = wk.y;
+    const int R = A.R, P = A.P, n = R - 2, K = A.K;
+    const int k_lo = kap * M, k_hi = min(k_lo + M, K) - 1, k_top = k_lo + M - 1;
+    const int gi_lo = GS_ROWS * I - k_top, gj_lo = TW * J - k_top;  // global of X(0, 0)
+    const int bi_lo = gi_lo + 1, bj_lo = gj_lo + 1;                 // global of Bs(0, 0)
+    const int tid = threadIdx.x;
+    float* __restrict__ x = A.x;
+    const Ring rg = A.ring;
+    const bool zero = A.zero_start != 0;
+
+    // ---- load (all loads of a batch in flight before any LDS store)
+    {
+        float v[NX];
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int idx = min(tid + q * NT, XH * XW - 1);
+            const int gi = min(max(gi_lo + idx / XW, 0), R - 1), gj = min(max(gj_lo + idx % XW, 0), R - 1);
+            v[q] = x[(size_t)gi * P + gj];
+        }
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int idx = tid + q * NT;
+            if (idx < XH * XW) X[(idx / XW) * XP + idx % XW] = v[q];
+        }
+    }
+    {
+        float v[NBX];
+#pragma unroll
+        for (int q = 0; q < NBX; ++q) {
+            const int idx = min(tid + q * NT, BH * BW - 1);
+            const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
+            v[q] = A.b[(size_t)gi * P + gj];
+        }
+#pragma unroll
+        for (int q = 0; q < NBX; ++q) {
+            const int idx = tid + q * NT;
+            if (idx < BH * BW) Bs[(idx / BW) * BP + idx % BW] = v[q];
+        }
+    }
+
+    // ---- lockstep sweep
+    const int w = tid >> 6, r = tid & 63;
+    const int k = k_lo + w;
+    const int i = GS_ROWS * I + 1 + r - k;
+    const bool row_ok = (k <= k_hi) && i >= 1 && i <= n;
+    const bool top = (i == 1), bot = (i == n), kpos = (k > 0), zero0 = zero && k == 0;
+    const int jbase = TW * J + 1 - k;
+    const int ofs = M - w;  // X column of skewed column 0 is 1 + (M - 1 - w)
+    float* const xrow = X + (1 + r + (M - 1 - w)) * XP;
+    const float* const brow = Bs + (r + (M - 1 - w)) * BP;
+    const float a = A.a, denom = A.denom;
+    float xprev = 0.f;
+    for (int s = 0; s < GS_ROWS + TW - 1; ++s) {
+        __syncthreads();
+        const int c = s - r;
+        const int j = jbase + c;
+        const bool active = row_ok && c >= 0 && c < TW && j >= 1 && j <= n;
+        const int lj = min(max(c + ofs, 1), XW - 2);
+        const int bj = min(max(c + ofs - 1, 0), BW - 1);
+        const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+        const float s_lds = xrow[lj + XP];
+        const float e_lds = xrow[lj + 1];
+        const float bb = brow[bj];
+        const bool left = (j == 1), right = (j == n);
+        float self_old = 0.f, n_lds = 0.f, w_lds = 0.f;
+        if (top || bot || left || right) self_old = xrow[lj];
+        if (r == 0 || top) n_lds = xrow[lj - XP];
+        if (c == 0 || left) w_lds = xrow[lj - 1];
+        const float tbv = sgn(rg.neg_tb, self_old), lrv = sgn(rg.neg_lr, self_old);
+        const float nv = top ? (kpos ? tbv : (zero ? 0.f : n_lds)) : (r == 0 ? n_lds : from_up);
+        const float wv = left ? (kpos ? lrv : (zero ? 0.f : w_lds)) : (c == 0 ? w_lds : xprev);
+        const float sv = bot ? (kpos ? tbv : (zero ? 0.f : s_lds)) : (zero0 ? 0.f : s_lds);
+        const float ev = right ? (kpos ? lrv : (zero ? 0.f : e_lds)) : (zero0 ? 0.f : e_lds);
+        const float v = relax_k<KIND>(bb, nv, sv, wv, ev, a, denom);
+        if (active) xrow[lj] = v;
+        xprev = active ? v : xprev;
+    }
+    __syncthreads();
+
+    // ---- write back every cell this unit relaxed (its latest sweep); cells
+    // relaxed at the solve's last sweep also write the ring they border.
+    for (int idx = tid; idx < XH * XW; idx += NT) {
+        const int li = idx / XW, lj = idx % XW;
+        const int gi = gi_lo + li, gj = gj_lo + lj;
+        const int lo = max(k_lo, max(GS_ROWS * I + 1 - gi, TW * J + 1 - gj));
+        const int hi = min(k_hi, min(GS_ROWS * I + GS_ROWS - gi, TW * J + TW - gj));
+        if (gi < 1 || gi > n || gj < 1 || gj > n || lo > hi) continue;
+        const float v = X[li * XP + lj];
+        const size_t o = (size_t)gi * P + gj;
+        x[o] = v;
+        if (hi == K - 1) {
+            const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
+            const float cn = 0.5f * (tb + lr);
+            if (gi == 1) x[o - P] = tb;
+            if (gi == n) x[o + P] = tb;
+            if (gj == 1) x[o - 1] = lr;
+            if (gj == n) x[o + 1] = lr;
+            if (gi == 1 && gj == 1) x[o - P - 1] = cn;
+            if (gi == 1 && gj == n) x[o - P + 1] = cn;
+            if (gi == n && gj == 1) x[o + P - 1] = cn;
+            if (gi == n && gj == n) x[o + P + 1] = cn;
         }
     }
 }
@@ -143,69 +362,88 @@ constexpr int JX = 64, JY = 32, JT = 256;
 template <int M>
 __global__ __launch_bounds__(JT) void k_jacobi_block(SweepArgs A, const float* __restrict__ src,
                                                     float* __restrict__ dst, int zero_src) {
-    constexpr int W = JX + 2 * M, H = JY + 2 * M;          // iterate region (tile + M halo)
+    constexpr int W = JX + 2 * M, H = JY + 2 * M;                // iterate region (tile + M halo)
     constexpr int BW = JX + 2 * (M - 1), BH = JY + 2 * (M - 1);  // right-hand side region
-    __shared__ float buf[2][H][W];
-    __shared__ float rhs[BH][BW];
+    constexpr int NX = (H * W + JT - 1) / JT, NBX = (BH * BW + JT - 1) / JT;
+    __shared__ float buf[2][H * W];
+    __shared__ float rhs[BH * BW];
 
     const int R = A.R, P = A.P;
     const int gi0 = 1 + JY * blockIdx.y - M, gj0 = 1 + JX * blockIdx.x - M;  // global of buf(0, 0)
     const int tid = threadIdx.x;
     const Ring rg = A.ring;
 
-    for (int idx = tid; idx < H * W; idx += JT) {
-        const int r = idx / W, c = idx % W;
-        const int gi = gi0 + r, gj = gj0 + c;
-        float val = 0.f;
-        if (!zero_src && gi >= 0 && gi < R && gj >= 0 && gj < R) val = src[(size_t)gi * P + gj];
-        buf[0][r][c] = val;
+    // all loads of the region are issued before any LDS store
+    {
+        float vals[NX];
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int idx = tid + q * JT;
+            const int r = idx / W, c = idx % W;
+            const int gi = min(max(gi0 + r, 0), R - 1), gj = min(max(gj0 + c, 0), R - 1);
+            vals[q] = zero_src ? 0.f : src[(size_t)gi * P + gj];
+        }
+#pragma unroll
+        for (int q = 0; q < NX; ++q)
+            if (tid + q * JT < H * W) buf[0][tid + q * JT] = vals[q];
     }
-    for (int idx = tid; idx < BH * BW; idx += JT) {
-        const int r = idx / BW, c = idx % BW;
-        const int gi = gi0 + 1 + r, gj = gj0 + 1 + c;
-        rhs[r][c] = (gi >= 1 && gi <= R - 2 && gj >= 1 && gj <= R - 2) ? A.b[(size_t)gi * P + gj] : 0.f;
+    {
+        float vals[NBX];
+#pragma unroll
+        for (int q = 0; q < NBX; ++q) {
+            const int idx = tid + q * JT;
+            const int r = idx / BW, c = idx % BW;
+            const int gi = min(max(gi0 + 1 + r, 1), R - 2), gj = min(max(gj0 + 1 + c, 1), R - 2);
+            vals[q] = A.b[(size_t)gi * P + gj];
+        }
+#pragma unroll
+        for (int q = 0; q < NBX; ++q)
+            if (tid + q * JT < BH * BW) rhs[tid + q * JT] = vals[q];
     }
     __syncthreads();
 
     const int kind = A.kind;
     const float a = A.a, denom = A.denom;
-#pragma unroll 1
+#pragma unroll
     for (int s = 1; s <= M; ++s) {
+        constexpr int dummy = 0;
+        (void)dummy;
         const int e = M - s;  // the region relaxed by sweep s is the tile grown by e
         const int rw = JX + 2 * e, rh = JY + 2 * e;
-        const float(*cur)[W] = buf[(s - 1) & 1];
-        float(*nxt)[W] = buf[s & 1];
+        const float* cur = buf[(s - 1) & 1];
+        float* nxt = buf[s & 1];
         const bool last = (s == M);
         for (int idx = tid; idx < rh * rw; idx += JT) {
             const int r = M - e + idx / rw, c = M - e + idx % rw;  // buf coordinates
             const int gi = gi0 + r, gj = gj0 + c;
             if (gi < 1 || gi > R - 2 || gj < 1 || gj > R - 2) continue;
-            const float v = relax(kind, rhs[r - 1][c - 1], cur[r - 1][c], cur[r + 1][c], cur[r][c - 1],
-                                  cur[r][c + 1], a, denom);
+            const int o = r * W + c;
+            const float v = relax(kind, rhs[(r - 1) * BW + c - 1], cur[o - W], cur[o + W], cur[o - 1], cur[o + 1], a,
+                                  denom);
             const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
             const float cn = 0.5f * (tb + lr);
             if (!last) {
-                nxt[r][c] = v;
+                nxt[o] = v;
                 // ring cells of this sweep (their inner neighbour is in the region)
-                if (gi == 1) nxt[r - 1][c] = tb;
-                if (gi == R - 2) nxt[r + 1][c] = tb;
-                if (gj == 1) nxt[r][c - 1] = lr;
-                if (gj == R - 2) nxt[r][c + 1] = lr;
-                if (gi == 1 && gj == 1) nxt[r - 1][c - 1] = cn;
-                if (gi == 1 && gj == R - 2) nxt[r - 1][c + 1] = cn;
-                if (gi == R - 2 && gj == 1) nxt[r + 1][c - 1] = cn;
-                if (gi == R - 2 && gj == R - 2) nxt[r + 1][c + 1] = cn;
+                if (gi == 1) nxt[o - W] = tb;
+                if (gi == R - 2) nxt[o + W] = tb;
+                if (gj == 1) nxt[o - 1] = lr;
+                if (gj == R - 2) nxt[o + 1] = lr;
+                if (gi == 1 && gj == 1) nxt[o - W - 1] = cn;
+                if (gi == 1 && gj == R - 2) nxt[o - W + 1] = cn;
+                if (gi == R - 2 && gj == 1) nxt[o + W - 1] = cn;
+                if (gi == R - 2 && gj == R - 2) nxt[o + W + 1] = cn;
             } else {
-                const size_t o = (size_t)gi * P + gj;
-                dst[o] = v;
-                if (gi == 1) dst[o - P] = tb;
-                if (gi == R - 2) dst[o + P] = tb;
-                if (gj == 1) dst[o - 1] = lr;
-                if (gj == R - 2) dst[o + 1] = lr;
-                if (gi == 1 && gj == 1) dst[o - P - 1] = cn;
-                if (gi == 1 && gj == R - 2) dst[o - P + 1] = cn;
-                if (gi == R - 2 && gj == 1) dst[o + P - 1] = cn;
-                if (gi == R - 2 && gj == R - 2) dst[o + P + 1] = cn;
+                const size_t g = (size_t)gi * P + gj;
+                dst[g] = v;
+                if (gi == 1) dst[g - P] = tb;
+                if (gi == R - 2) dst[g + P] = tb;
+                if (gj == 1) dst[g - 1] = lr;
+                if (gj == R - 2) dst[g + 1] = lr;
+                if (gi == 1 && gj == 1) dst[g - P - 1] = cn;
+                if (gi == 1 && gj == R - 2) dst[g - P + 1] = cn;
+                if (gi == R - 2 && gj == 1) dst[g + P - 1] = cn;
+                if (gi == R - 2 && gj == R - 2) dst[g + P + 1] = cn;
             }
         }
         __syncthreads();
@@ -239,33 +477,85 @@ hipError_t launch_jacobi_block(hipStream_t s, const SweepArgs& a, const float* s
 // ---- Gauss-Seidel plan ----------------------------------------------------
 hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW) {
     plan->TW = TW;
+    plan->M = 0;
     plan->GI = (R - 2 + GS_ROWS - 1) / GS_ROWS;
     plan->GJ = (R - 2 + TW - 1) / TW;
     plan->K = K;
-    plan->levels = (K > 0) ? (plan->GI - 1) + (plan->GJ - 1) + 2 * (K - 1) + 1 : 0;
+    const int raw_levels = (K > 0) ? (plan->GI - 1) + (plan->GJ - 1) + 2 * (K - 1) + 1 : 0;
+    plan->levels = 0;
     plan->work = nullptr;
     plan->host_offsets = nullptr;
     if (plan->GI >= (1 << 15) || plan->GJ >= (1 << 16)) return hipErrorInvalidValue;
-    std::vector<int> count(plan->levels + 1, 0);
+    // Tile-sweeps per hyperplane level; empty levels (e.g. odd levels of a
+    // one-tile grid) are dropped, which keeps the order of the others.
+    std::vector<int> count(raw_levels, 0);
     for (int k = 0; k < K; ++k)
         for (int I = 0; I < plan->GI; ++I)
             for (int J = 0; J < plan->GJ; ++J) ++count[I + J + 2 * k];
-    plan->host_offsets = new int[plan->levels + 1];
+    std::vector<int> slot(raw_levels, -1);
+    std::vector<int> offsets;
     int acc = 0;
-    for (int L = 0; L < plan->levels; ++L) {
-        plan->host_offsets[L] = acc;
+    for (int L = 0; L < raw_levels; ++L) {
+        if (count[L] == 0) continue;
+        slot[L] = static_cast<int>(offsets.size());
+        offsets.push_back(acc);
         acc += count[L];
     }
-    plan->host_offsets[plan->levels] = acc;
+    plan->levels = static_cast<int>(offsets.size());
+    offsets.push_back(acc);
+    plan->host_offsets = new int[offsets.size()];
+    std::copy(offsets.begin(), offsets.end(), plan->host_offsets);
     if (acc == 0) return hipSuccess;
     std::vector<int2> work(acc);
-    std::vector<int> fill(plan->host_offsets, plan->host_offsets + plan->levels);
+    std::vector<int> fill(offsets.begin(), offsets.end() - 1);
     for (int k = 0; k < K; ++k)
         for (int I = 0; I < plan->GI; ++I)
-            for (int J = 0; J < plan->GJ; ++J) work[fill[I + J + 2 * k]++] = make_int2((I << 16) | J, k);
+            for (int J = 0; J < plan->GJ; ++J) work[fill[slot[I + J + 2 * k]]++] = make_int2((I << 16) | J, k);
     hipError_t e = hipMalloc(&plan->work, sizeof(int2) * acc);
     if (e != hipSuccess) return e;
     return hipMemcpy(plan->work, work.data(), sizeof(int2) * acc, hipMemcpyHostToDevice);
+}
+
+hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M) {
+    const int n = R - 2;
+    plan->TW = TW;
+    plan->M = M;
+    plan->GI = (n + K - 1 + GS_ROWS - 1) / GS_ROWS;
+    plan->GJ = (n + K - 1 + TW - 1) / TW;
+    plan->K = K;
+    plan->levels = 0;
+    plan->work = nullptr;
+    plan->host_offsets = nullptr;
+    if (plan->GI >= (1 << 15) || plan->GJ >= (1 << 16)) return hipErrorInvalidValue;
+    const int GK = (K + M - 1) / M;
+    // units (I', J', kappa) holding at least one interior cell, by level I' + J' + kappa
+    std::vector<std::vector<int2>> by_level(K > 0 ? plan->GI + plan->GJ + GK : 0);
+    for (int kap = 0; kap < GK; ++kap) {
+        const int k_lo = kap * M, k_hi = std::min(k_lo + M, K) - 1;
+        for (int I = 0; I < plan->GI; ++I)
+            for (int J = 0; J < plan->GJ; ++J) {
+                bool any = false;
+                for (int k = k_lo; k <= k_hi && !any; ++k)
+                    any = GS_ROWS * I + 1 <= n + k && GS_ROWS * I + GS_ROWS >= 1 + k && TW * J + 1 <= n + k &&
+                          TW * J + TW >= 1 + k;
+                if (any) by_level[I + J + kap].push_back(make_int2((I << 16) | J, kap));
+            }
+    }
+    std::vector<int> offsets;
+    std::vector<int2> work;
+    for (auto& lv : by_level) {
+        if (lv.empty()) continue;
+        offsets.push_back(static_cast<int>(work.size()));
+        work.insert(work.end(), lv.begin(), lv.end());
+    }
+    plan->levels = static_cast<int>(offsets.size());
+    offsets.push_back(static_cast<int>(work.size()));
+    plan->host_offsets = new int[offsets.size()];
+    std::copy(offsets.begin(), offsets.end(), plan->host_offsets);
+    if (work.empty()) return hipSuccess;
+    hipError_t e = hipMalloc(&plan->work, sizeof(int2) * work.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(plan->work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice);
 }
 
 void gs_plan_destroy(GsPlan* plan) {
@@ -275,17 +565,41 @@ void gs_plan_destroy(GsPlan* plan) {
     plan->host_offsets = nullptr;
 }
 
+template <int M>
+void launch_unit(hipStream_t s, int n, const int2* w, const SweepArgs& a) {
+    if (a.kind == SWEEP_DIFFUSE)
+        hipLaunchKernelGGL((k_gs_unit<64, M, SWEEP_DIFFUSE>), dim3(n), dim3(64 * M), 0, s, a, w);
+    else
+        hipLaunchKernelGGL((k_gs_unit<64, M, SWEEP_PRESSURE>), dim3(n), dim3(64 * M), 0, s, a, w);
+}
+
 hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a) {
     if (plan.K != a.K) return hipErrorInvalidValue;
     for (int L = 0; L < plan.levels; ++L) {
         const int n = plan.host_offsets[L + 1] - plan.host_offsets[L];
+        if (n <= 0) continue;
         const int2* w = plan.work + plan.host_offsets[L];
-        if (plan.TW == 64)
-            hipLaunchKernelGGL(k_gs_tile<64>, dim3(n), dim3(64), 0, s, a, w);
-        else if (plan.TW == 128)
-            hipLaunchKernelGGL(k_gs_tile<128>, dim3(n), dim3(64), 0, s, a, w);
-        else
-            return hipErrorInvalidValue;
+        const bool dif = a.kind == SWEEP_DIFFUSE;
+        if (plan.M == 0) {
+            if (plan.TW == 64 && dif)
+                hipLaunchKernelGGL((k_gs_tile<64, SWEEP_DIFFUSE>), dim3(n), dim3(64), 0, s, a, w);
+            else if (plan.TW == 64)
+                hipLaunchKernelGGL((k_gs_tile<64, SWEEP_PRESSURE>), dim3(n), dim3(64), 0, s, a, w);
+            else if (plan.TW == 128 && dif)
+                hipLaunchKernelGGL((k_gs_tile<128, SWEEP_DIFFUSE>), dim3(n), dim3(64), 0, s, a, w);
+            else if (plan.TW == 128)
+                hipLaunchKernelGGL((k_gs_tile<128, SWEEP_PRESSURE>), dim3(n), dim3(64), 0, s, a, w);
+            else
+                return hipErrorInvalidValue;
+        } else {
+            if (plan.TW != 64) return hipErrorInvalidValue;
+            switch (plan.M) {
+                case 2: launch_unit<2>(s, n, w, a); break;
+                case 4: launch_unit<4>(s, n, w, a); break;
+                case 8: launch_unit<8>(s, n, w, a); break;
+                default: return hipErrorInvalidValue;
+            }
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -59,7 +59,8 @@ typedef struct fs2d_params {
     int sweep_order;         /* enum fs2d_sweep_order                                                   */
     int jacobi_block;        /* Jacobi sweeps fused per launch (temporal block depth); 0 = autotuned    */
     int gs_tile_cols;        /* Gauss-Seidel tile width in cells; 0 = default                            */
-    int reserved[3];         /* must be 0                                                                */
+    int gs_block;            /* Gauss-Seidel sweeps per tile (temporal block): 1, 2, 4 or 8; 0 = default */
+    int reserved[2];         /* must be 0                                                                */
 } fs2d_params;
 
 typedef struct fs2d_handle fs2d_handle;

@@ -216,3 +216,151 @@ def jacobi_block_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, J
         zero = False
         done += m
     return src
+
+
+# ---------------------------------------------------------------------------
+# Gauss-Seidel, temporally blocked: k_gs_unit (m sweeps per tile in skewed
+# coordinates i' = i + k, j' = j + k; wave w of a unit relaxes sweep
+# kappa*m + w, all waves on the same skewed anti-diagonal per step).
+def gs_unit_plan(R, K, TW, M):
+    n = R - 2
+    GI = -(-(n + K - 1) // ROWS)
+    GJ = -(-(n + K - 1) // TW)
+    GK = -(-K // M)
+    levels = {}
+    for kap in range(GK):
+        k_lo, k_hi = kap * M, min(kap * M + M, K) - 1
+        for I in range(GI):
+            for J in range(GJ):
+                ok = False
+                for k in range(k_lo, k_hi + 1):
+                    if ROWS * I + 1 <= n + k and ROWS * I + ROWS >= 1 + k and \
+                       TW * J + 1 <= n + k and TW * J + TW >= 1 + k:
+                        ok = True
+                        break
+                if ok:
+                    levels.setdefault(I + J + kap, []).append((I, J, kap))
+    return [levels[L] for L in sorted(levels)]
+
+
+def _unit_computed(gi, gj, I, J, k_lo, k_hi, n, TW):
+    lo = max(k_lo, ROWS * I + 1 - gi, TW * J + 1 - gj)
+    hi = min(k_hi, ROWS * I + ROWS - gi, TW * J + TW - gj)
+    inside = 1 <= gi <= n and 1 <= gj <= n
+    return inside and lo <= hi, hi
+
+
+def gs_unit_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, TW=64):
+    """In-place Gauss-Seidel solve by the temporally blocked unit schedule."""
+    R = x.shape[0]
+    n = R - 2
+    a, denom = F(a), F(denom)
+    XH, XW = ROWS + 1 + M, TW + 1 + M
+    for level in gs_unit_plan(R, K, TW, M):
+        snap = x.copy()
+        reads, writes, updates = [], [], []
+        for (I, J, kap) in level:
+            rd, wr = set(), set()
+            k_lo, k_hi, k_top = kap * M, min(kap * M + M, K) - 1, kap * M + M - 1
+            gi_lo, gj_lo = ROWS * I - k_top, TW * J - k_top
+            X = np.zeros((XH, XW), np.float32)
+            for li in range(XH):
+                for lj in range(XW):
+                    X[li, lj] = snap[min(max(gi_lo + li, 0), R - 1), min(max(gj_lo + lj, 0), R - 1)]
+
+            def use(li, lj):
+                gi, gj = gi_lo + li, gj_lo + lj
+                assert 0 <= li < XH and 0 <= lj < XW and 0 <= gi < R and 0 <= gj < R, (li, lj, gi, gj)
+                rd.add((gi, gj))
+                return X[li, lj]
+
+            xprev = np.zeros((M, ROWS), np.float32)
+            written_at = {}
+            for s in range(ROWS + TW - 1):
+                pending = []
+                step_reads = set()
+                for w in range(M):
+                    k = kap * M + w
+                    if k >= K:
+                        continue
+                    from_up = np.concatenate([[F(0)], xprev[w, :-1]])
+                    new_prev = xprev[w].copy()
+                    for r in range(ROWS):
+                        c = s - r
+                        i = ROWS * I + 1 + r - k
+                        j = TW * J + 1 + c - k
+                        if not (0 <= c < TW and 1 <= i <= n and 1 <= j <= n):
+                            continue
+                        li, lj = i - gi_lo, j - gj_lo
+                        zero0 = zero_start and k == 0
+                        self_old = use(li, lj)
+                        if i == 1:
+                            N = _sgn(neg_tb, self_old) if k > 0 else (F(0) if zero_start else use(li - 1, lj))
+                        elif r == 0:
+                            N = use(li - 1, lj)
+                        else:
+                            assert written_at.get((w, r - 1, c)) == s - 1
+                            N = from_up[r]
+                        if j == 1:
+                            Wv = _sgn(neg_lr, self_old) if k > 0 else (F(0) if zero_start else use(li, lj - 1))
+                        elif c == 0:
+                            Wv = use(li, lj - 1)
+                        else:
+                            assert written_at.get((w, r, c - 1)) == s - 1
+                            Wv = xprev[w, r]
+                        if i == n:
+                            S = _sgn(neg_tb, self_old) if k > 0 else (F(0) if zero_start else use(li + 1, lj))
+                        else:
+                            S = F(0) if zero0 else use(li + 1, lj)
+                        if j == n:
+                            E = _sgn(neg_lr, self_old) if k > 0 else (F(0) if zero_start else use(li, lj + 1))
+                        else:
+                            E = F(0) if zero0 else use(li, lj + 1)
+                        for (pi, pj) in ((li - 1, lj), (li + 1, lj), (li, lj - 1), (li, lj + 1)):
+                            step_reads.add((pi, pj))
+                        bb = b[i, j]
+                        v = _relax(kind, bb, N, S, Wv, E, a, denom)
+                        pending.append((li, lj, v))
+                        new_prev[r] = v
+                        written_at[(w, r, c)] = s
+                    xprev[w] = new_prev
+                # one barrier per step: no wave may write what another reads in
+                # the same step
+                wset = {(li, lj) for (li, lj, _) in pending}
+                assert len(wset) == len(pending)
+                assert not (step_reads & wset), ("in-step read/write overlap", s)
+                for (li, lj, v) in pending:
+                    X[li, lj] = v
+            out = {}
+            for li in range(XH):
+                for lj in range(XW):
+                    gi, gj = gi_lo + li, gj_lo + lj
+                    ok, khi = _unit_computed(gi, gj, I, J, k_lo, k_hi, n, TW)
+                    if not ok:
+                        continue
+                    out[(gi, gj)] = X[li, lj]
+                    if khi == K - 1:
+                        v = X[li, lj]
+                        tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                        cn = F(F(0.5) * F(tb + lr))
+                        if gi == 1: out[(0, gj)] = tb
+                        if gi == n: out[(n + 1, gj)] = tb
+                        if gj == 1: out[(gi, 0)] = lr
+                        if gj == n: out[(gi, n + 1)] = lr
+                        if gi == 1 and gj == 1: out[(0, 0)] = cn
+                        if gi == 1 and gj == n: out[(0, n + 1)] = cn
+                        if gi == n and gj == 1: out[(n + 1, 0)] = cn
+                        if gi == n and gj == n: out[(n + 1, n + 1)] = cn
+            wr.update(out.keys())
+            reads.append(rd)
+            writes.append(wr)
+            updates.append(out)
+        for t in range(len(level)):
+            for u in range(len(level)):
+                if t != u:
+                    assert not (reads[t] & writes[u]), ("read/write overlap", level[t], level[u])
+                    assert not (writes[t] & writes[u]), ("write/write overlap", level[t], level[u])
+        for out in updates:
+            for (i, j), val in out.items():
+                x[i, j] = val
+    return x

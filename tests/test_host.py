@@ -15,7 +15,7 @@ import fs2d_amd
 import oracle
 from conftest import ROOT, bits_equal
 from fs2d_amd import workloads as W
-from schedule_models import gs_tile_model, jacobi_block_model
+from schedule_models import gs_tile_model, gs_unit_model, jacobi_block_model
 
 
 def header_symbols():
@@ -48,6 +48,7 @@ def test_default_params_are_the_reference_literals():
     (10, 10, {"order": 7}, "sweep_order"),
     (10, 10, {"jacobi_block": 9}, "jacobi_block"),
     (10, 10, {"gs_tile_cols": 48}, "gs_tile_cols"),
+    (10, 10, {"gs_block": 3}, "gs_block"),
 ])
 def test_create_validates_arguments_before_touching_the_device(rows, cols, kw, msg):
     with pytest.raises(fs2d_amd.Fs2dError, match=msg):
@@ -77,6 +78,27 @@ def test_gs_tile_schedule_reproduces_gauss_seidel(R, TW, oracle_lib):
     p_ref = oracle_lib.project(a, b, 3, oracle.GS_LEX, want_p=True)
     p = np.full((R, R), 7.0, np.float32)  # garbage must never be read at sweep 0
     gs_tile_model(p, d, 3, 1, 0, 0, False, False, 1, TW)
+    assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M", [(3, 2), (10, 4), (67, 2), (70, 8)])
+def test_gs_unit_schedule_reproduces_gauss_seidel(R, M, oracle_lib):
+    """Temporally blocked Gauss-Seidel (k_gs_unit): M sweeps per skewed tile,
+    one wave per sweep in lockstep, in place; bit-identical to GS-lex."""
+    rho, u, v, *_ = W.random_state(R, seed=R + 2)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        for K in (1, 5, 9):
+            x = rho.copy()
+            gs_unit_model(x, rho.copy(), K, 0, a, den, mode == 2, mode == 1, 0, M)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert bits_equal(x, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
+    p = np.full((R, R), 7.0, np.float32)
+    gs_unit_model(p, d, 9, 1, 0, 0, False, False, 1, M)
     assert bits_equal(p, p_ref)
 
 
