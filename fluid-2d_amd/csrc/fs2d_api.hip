@@ -98,7 +98,8 @@ hipError_t gs_plan_for(fs2d_handle* h, int K, const GsPlan** out) {
 int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, float denom, int mode, int K,
                  int zero_start) {
     if (K <= 0) return FS2D_OK;
-    SweepArgs A{x, b, h->R, h->P, kind, a, denom, ring_of(mode), K, zero_start};
+    SweepArgs A{x, b, h->R, h->P, kind, a, denom, kind == SWEEP_DIFFUSE ? 1.0f / denom : 0.f, ring_of(mode), K,
+                zero_start};
     if (h->prm.sweep_order == FS2D_ORDER_GS_LEX) {
         const GsPlan* plan = nullptr;
         FS2D_HIP(h, gs_plan_for(h, K, &plan));

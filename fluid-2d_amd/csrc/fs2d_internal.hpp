@@ -40,6 +40,7 @@ struct SweepArgs {
     int R, P;
     int kind;        // SweepKind
     float a, denom;  // diffuse: a and 1 + 4a
+    float inv_denom; // RN(1 / denom), for the FMA-corrected division
     Ring ring;
     int K;           // total sweeps of this solve
     int zero_start;  // iterate starts at 0 everywhere, ring included (pressure, :186)

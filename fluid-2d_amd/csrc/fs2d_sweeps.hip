@@ -36,10 +36,26 @@ namespace {
 
 __device__ __forceinline__ float sgn(bool neg, float x) { return neg ? -x : x; }
 
+// x / d correctly rounded, given y = RN(1/d): q0 = RN(x y) is within ~1.5 ulp,
+// one FMA correction makes it faithful, and Markstein's step from a faithful
+// quotient (remainder exact by FMA) rounds correctly (P. Markstein, IBM J. R&D
+// 34(1), 1990; Muller et al., Handbook of Floating-Point Arithmetic).  Dividends
+// outside [2^-100, 2^100] (zero, tiny, huge, inf, NaN) take the IEEE division,
+// so no intermediate underflows or overflows.  Pinned against exact rational
+// arithmetic in tests/test_division.py.
+__device__ __forceinline__ float div_rn(float x, float d, float y) {
+    const float ax = fabsf(x);
+    if (!(ax >= 0x1p-100f && ax <= 0x1p100f)) return x / d;
+    const float q0 = x * y;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, d, x), y, q0);
+    return __builtin_fmaf(__builtin_fmaf(-q1, d, x), y, q1);
+}
+
 // The update of one cell; operand order as in src/fluid_solver_cpu.cpp:107-108
 // (diffuse: N + S + W + E) and :199-200 (pressure: d + E + W + S + N).
-__device__ __forceinline__ float relax(int kind, float b, float n, float s, float w, float e, float a, float denom) {
-    if (kind == SWEEP_DIFFUSE) return (b + a * (((n + s) + w) + e)) / denom;
+__device__ __forceinline__ float relax(int kind, float b, float n, float s, float w, float e, float a, float denom,
+                                       float inv) {
+    if (kind == SWEEP_DIFFUSE) return div_rn(b + a * (((n + s) + w) + e), denom, inv);
     return ((((b + e) + w) + s) + n) / 4.0f;
 }
 
@@ -68,8 +84,9 @@ __device__ __forceinline__ const float* ring_source(const float* x, int gi, int 
 }
 
 template <int KIND>
-__device__ __forceinline__ float relax_k(float b, float n, float s, float w, float e, float a, float denom) {
-    if (KIND == SWEEP_DIFFUSE) return (b + a * (((n + s) + w) + e)) / denom;
+__device__ __forceinline__ float relax_k(float b, float n, float s, float w, float e, float a, float denom,
+                                         float inv) {
+    if (KIND == SWEEP_DIFFUSE) return div_rn(b + a * (((n + s) + w) + e), denom, inv);
     return ((((b + e) + w) + s) + n) / 4.0f;
 }
 
@@ -170,7 +187,7 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
     // lane r + 1 overwrites cell (r + 1, c + 1) only at step s + 2), so only
     // xprev and the DPP shift carry the dependency from step to step.
     // Inactive lanes compute on clamped operands and discard the result.
-    const float a = A.a, denom = A.denom;
+    const float a = A.a, denom = A.denom, inv = A.inv_denom;
     const int steps = rows + cols - 1;
     const bool lane_ok = lane < rows;
     const float w_halo = T[(lane + 1) * TP];
@@ -192,7 +209,7 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
         const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
         const float n = (lane == 0) ? n_halo : from_up;
         const float w = (c == 0) ? w_halo : xprev;
-        const float v = relax_k<KIND>(bb, n, so, w, e, a, denom);
+        const float v = relax_k<KIND>(bb, n, so, w, e, a, denom, inv);
         if (active) trow[cw + 1] = v;
         xprev = active ? v : xprev;
     }
@@ -247,8 +264,9 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
     constexpr int BH = GS_ROWS - 1 + M, BW = TW - 1 + M;             // right-hand side region
     constexpr int BP = BW + (BW & 1);
     constexpr int NX = (XH * XW + NT - 1) / NT, NBX = (BH * BW + NT - 1) / NT;
-    __shared__ float X[XH * XP];
-    __shared__ float Bs[BH * BP];
+    constexpr int GUARD = GS_ROWS + TW + M;  // reads of inactive lanes in the plain sweep
+    __shared__ float X[XH * XP + GUARD];
+    __shared__ float Bs[BH * BP + GUARD];
 
     const int2 wk = work[blockIdx.x];
     const int I = wk.x >> 16, J = wk.x & 0xffff, kap = wk.y;
@@ -295,38 +313,63 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
     const int w = tid >> 6, r = tid & 63;
     const int k = k_lo + w;
     const int i = GS_ROWS * I + 1 + r - k;
-    const bool row_ok = (k <= k_hi) && i >= 1 && i <= n;
-    const bool top = (i == 1), bot = (i == n), kpos = (k > 0), zero0 = zero && k == 0;
+    const bool wave_ok = (k <= k_hi);
     const int jbase = TW * J + 1 - k;
     const int ofs = M - w;  // X column of skewed column 0 is 1 + (M - 1 - w)
     float* const xrow = X + (1 + r + (M - 1 - w)) * XP;
     const float* const brow = Bs + (r + (M - 1 - w)) * BP;
-    const float a = A.a, denom = A.denom;
+    const float a = A.a, denom = A.denom, inv = A.inv_denom;
     float xprev = 0.f;
-    for (int s = 0; s < GS_ROWS + TW - 1; ++s) {
-        __syncthreads();
-        const int c = s - r;
-        const int j = jbase + c;
-        const bool active = row_ok && c >= 0 && c < TW && j >= 1 && j <= n;
-        const int lj = min(max(c + ofs, 1), XW - 2);
-        const int bj = min(max(c + ofs - 1, 0), BW - 1);
-        const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-        const float s_lds = xrow[lj + XP];
-        const float e_lds = xrow[lj + 1];
-        const float bb = brow[bj];
-        const bool left = (j == 1), right = (j == n);
-        float self_old = 0.f, n_lds = 0.f, w_lds = 0.f;
-        if (top || bot || left || right) self_old = xrow[lj];
-        if (r == 0 || top) n_lds = xrow[lj - XP];
-        if (c == 0 || left) w_lds = xrow[lj - 1];
-        const float tbv = sgn(rg.neg_tb, self_old), lrv = sgn(rg.neg_lr, self_old);
-        const float nv = top ? (kpos ? tbv : (zero ? 0.f : n_lds)) : (r == 0 ? n_lds : from_up);
-        const float wv = left ? (kpos ? lrv : (zero ? 0.f : w_lds)) : (c == 0 ? w_lds : xprev);
-        const float sv = bot ? (kpos ? tbv : (zero ? 0.f : s_lds)) : (zero0 ? 0.f : s_lds);
-        const float ev = right ? (kpos ? lrv : (zero ? 0.f : e_lds)) : (zero0 ? 0.f : e_lds);
-        const float v = relax_k<KIND>(bb, nv, sv, wv, ev, a, denom);
-        if (active) xrow[lj] = v;
-        xprev = active ? v : xprev;
+    // A unit whose cells all lie in rows and columns 2..n-1 and that does not
+    // start a zero-initialised solve never meets the ring: its sweep needs no
+    // boundary logic and no clamping (the LDS arrays carry guard space for the
+    // reads of inactive lanes, whose results are discarded).
+    const bool plain = !(zero && k_lo == 0) && GS_ROWS * I + 1 - k_hi >= 2 && GS_ROWS * I + GS_ROWS - k_lo <= n - 1 &&
+                       TW * J + 1 - k_hi >= 2 && TW * J + TW - k_lo <= n - 1;
+    if (plain) {
+        float* const xr = xrow + ofs - r;           // xr[s] = cell at skewed column s - r
+        const float* const br = brow + ofs - 1 - r;
+        for (int s = 0; s < GS_ROWS + TW - 1; ++s) {
+            __syncthreads();
+            const int c = s - r;
+            const bool active = wave_ok && (unsigned)c < (unsigned)TW;
+            const float so = xr[s + XP], e = xr[s + 1], bb = br[s];
+            const float nh = xr[s - XP], wh = xr[s - 1];
+            const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+            const float nv = (r == 0) ? nh : from_up;
+            const float wv = (c == 0) ? wh : xprev;
+            const float v = relax_k<KIND>(bb, nv, so, wv, e, a, denom, inv);
+            if (active) xr[s] = v;
+            xprev = active ? v : xprev;
+        }
+    } else {
+        const bool row_ok = wave_ok && i >= 1 && i <= n;
+        const bool top = (i == 1), bot = (i == n), kpos = (k > 0), zero0 = zero && k == 0;
+        for (int s = 0; s < GS_ROWS + TW - 1; ++s) {
+            __syncthreads();
+            const int c = s - r;
+            const int j = jbase + c;
+            const bool active = row_ok && c >= 0 && c < TW && j >= 1 && j <= n;
+            const int lj = min(max(c + ofs, 1), XW - 2);
+            const int bj = min(max(c + ofs - 1, 0), BW - 1);
+            const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+            const float s_lds = xrow[lj + XP];
+            const float e_lds = xrow[lj + 1];
+            const float bb = brow[bj];
+            const bool left = (j == 1), right = (j == n);
+            float self_old = 0.f, n_lds = 0.f, w_lds = 0.f;
+            if (top || bot || left || right) self_old = xrow[lj];
+            if (r == 0 || top) n_lds = xrow[lj - XP];
+            if (c == 0 || left) w_lds = xrow[lj - 1];
+            const float tbv = sgn(rg.neg_tb, self_old), lrv = sgn(rg.neg_lr, self_old);
+            const float nv = top ? (kpos ? tbv : (zero ? 0.f : n_lds)) : (r == 0 ? n_lds : from_up);
+            const float wv = left ? (kpos ? lrv : (zero ? 0.f : w_lds)) : (c == 0 ? w_lds : xprev);
+            const float sv = bot ? (kpos ? tbv : (zero ? 0.f : s_lds)) : (zero0 ? 0.f : s_lds);
+            const float ev = right ? (kpos ? lrv : (zero ? 0.f : e_lds)) : (zero0 ? 0.f : e_lds);
+            const float v = relax_k<KIND>(bb, nv, sv, wv, ev, a, denom, inv);
+            if (active) xrow[lj] = v;
+            xprev = active ? v : xprev;
+        }
     }
     __syncthreads();
 
@@ -403,7 +446,7 @@ __global__ __launch_bounds__(JT) void k_jacobi_block(SweepArgs A, const float* _
     __syncthreads();
 
     const int kind = A.kind;
-    const float a = A.a, denom = A.denom;
+    const float a = A.a, denom = A.denom, inv = A.inv_denom;
 #pragma unroll
     for (int s = 1; s <= M; ++s) {
         constexpr int dummy = 0;
@@ -419,7 +462,7 @@ __global__ __launch_bounds__(JT) void k_jacobi_block(SweepArgs A, const float* _
             if (gi < 1 || gi > R - 2 || gj < 1 || gj > R - 2) continue;
             const int o = r * W + c;
             const float v = relax(kind, rhs[(r - 1) * BW + c - 1], cur[o - W], cur[o + W], cur[o - 1], cur[o + 1], a,
-                                  denom);
+                                  denom, inv);
             const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
             const float cn = 0.5f * (tb + lr);
             if (!last) {
