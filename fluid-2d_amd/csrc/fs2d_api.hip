@@ -38,6 +38,7 @@ struct fs2d_handle {
     std::map<int, GsPlan> gs_plans;  // by sweep count
     int gs_tw = 64;
     int gs_block = 8;
+    int gs_lag = 8;
     int jacobi_block = 4;
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
     std::string err;
@@ -81,7 +82,7 @@ hipError_t gs_plan_for(fs2d_handle* h, int K, const GsPlan** out) {
     auto it = h->gs_plans.find(K);
     if (it == h->gs_plans.end()) {
         GsPlan plan{};
-        hipError_t e = (h->gs_block > 1) ? gs_unit_plan_create(&plan, h->R, K, h->gs_tw, h->gs_block)
+        hipError_t e = (h->gs_block > 1) ? gs_unit_plan_create(&plan, h->R, K, h->gs_tw, h->gs_block, h->gs_lag)
                                          : gs_plan_create(&plan, h->R, K, h->gs_tw);
         if (e != hipSuccess) {
             gs_plan_destroy(&plan);
@@ -103,6 +104,11 @@ int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, f
     if (h->prm.sweep_order == FS2D_ORDER_GS_LEX) {
         const GsPlan* plan = nullptr;
         FS2D_HIP(h, gs_plan_for(h, K, &plan));
+        if (plan->M > 0 && zero_start) {
+            // the unit kernel starts from a zero-filled iterate, ring included
+            FS2D_HIP(h, launch_fill(h->stream, x, 0u, h->field_bytes / 4));
+            A.zero_start = 0;
+        }
         FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
     } else {
         FS2D_HIP(h, launch_jacobi_solve(h->stream, A, h->f[F_SCRATCH], h->jacobi_block));
@@ -218,6 +224,8 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_tile_cols must be 0, 64 or 128");
     if (prm.gs_block != 0 && prm.gs_block != 1 && prm.gs_block != 2 && prm.gs_block != 4 && prm.gs_block != 8)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_block must be 0, 1, 2, 4 or 8");
+    if (prm.gs_lag != 0 && prm.gs_lag != 1 && prm.gs_lag != 4 && prm.gs_lag != 8)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_lag must be 0, 1, 4 or 8");
     if (prm.gs_block > 1 && prm.gs_tile_cols == 128)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_block > 1 needs gs_tile_cols 64");
     for (int r : prm.reserved)
@@ -230,6 +238,7 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     h->prm = prm;
     h->gs_tw = prm.gs_tile_cols ? prm.gs_tile_cols : 64;
     h->gs_block = prm.gs_block ? prm.gs_block : 8;
+    h->gs_lag = prm.gs_lag ? prm.gs_lag : 8;
     h->jacobi_block = prm.jacobi_block ? prm.jacobi_block : 4;
     auto bail = [&](hipError_t e, const char* what) {
         std::string msg = std::string("fs2d_hip_create: ") + what + ": " + hipGetErrorString(e);

@@ -64,6 +64,7 @@ hipError_t launch_fill(hipStream_t s, void* p, uint32_t value, size_t n_words);
 struct GsPlan {
     int TW;                // tile columns
     int M;                 // sweeps per unit (0: one sweep per tile, k_gs_tile)
+    int lag;               // steps between consecutive sweeps' waves (barrier interval)
     int GI, GJ;            // tile grid
     int K;
     int levels;
@@ -71,7 +72,7 @@ struct GsPlan {
     int* host_offsets;     // levels + 1 offsets into work
 };
 hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW);
-hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M);
+hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int lag);
 void gs_plan_destroy(GsPlan* plan);
 hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a);
 

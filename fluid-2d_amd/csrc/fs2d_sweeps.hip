@@ -256,7 +256,7 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
 // overwritten at step t + 2, so the M waves relax in lockstep, in place in LDS,
 // with one barrier per step.  Units with equal I' + J' + kappa touch disjoint
 // cells and are launched together.
-template <int TW, int M, int KIND>
+template <int TW, int M, int L, int KIND>
 __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __restrict__ work) {
     constexpr int NT = 64 * M;
     constexpr int XH = GS_ROWS + 1 + M, XW = TW + 1 + M;            // iterate region
@@ -277,7 +277,6 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
     const int tid = threadIdx.x;
     float* __restrict__ x = A.x;
     const Ring rg = A.ring;
-    const bool zero = A.zero_start != 0;
 
     // ---- load (all loads of a batch in flight before any LDS store)
     {
@@ -320,61 +319,101 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
     const float* const brow = Bs + (r + (M - 1 - w)) * BP;
     const float a = A.a, denom = A.denom, inv = A.inv_denom;
     float xprev = 0.f;
-    // A unit whose cells all lie in rows and columns 2..n-1 and that does not
-    // start a zero-initialised solve never meets the ring: its sweep needs no
-    // boundary logic and no clamping (the LDS arrays carry guard space for the
-    // reads of inactive lanes, whose results are discarded).
-    const bool plain = !(zero && k_lo == 0) && GS_ROWS * I + 1 - k_hi >= 2 && GS_ROWS * I + GS_ROWS - k_lo <= n - 1 &&
+    // A unit whose cells all lie in rows and columns 2..n-1 never meets the
+    // ring: its sweep needs no clamping (the LDS arrays carry guard space for
+    // the reads of inactive lanes, whose results are discarded).  The unit
+    // kernel expects a zero-start iterate to be zero-filled by the caller.
+    const bool plain = GS_ROWS * I + 1 - k_hi >= 2 && GS_ROWS * I + GS_ROWS - k_lo <= n - 1 &&
                        TW * J + 1 - k_hi >= 2 && TW * J + TW - k_lo <= n - 1;
+    // Lagged waves: wave w runs its local step s at tick s + L w, and a barrier
+    // opens every interval of L ticks.  An operand of step s from the previous
+    // sweep was written by wave w - 1 at its step s - 1, i.e. at tick
+    // s + L w - L - 1, before the interval's barrier, and no other wave writes
+    // a cell this wave reads within the interval; so each wave reads all
+    // operands of its L steps right after the barrier and then relaxes them
+    // with only the DPP / register chain between steps.
+    constexpr int STEPS = GS_ROWS + TW - 1;
+    constexpr int NINT = (STEPS + L * (M - 1) + L - 1) / L;
     if (plain) {
         float* const xr = xrow + ofs - r;           // xr[s] = cell at skewed column s - r
         const float* const br = brow + ofs - 1 - r;
-        for (int s = 0; s < GS_ROWS + TW - 1; ++s) {
+        for (int it = 0; it < NINT; ++it) {
             __syncthreads();
-            const int c = s - r;
-            const bool active = wave_ok && (unsigned)c < (unsigned)TW;
-            const float so = xr[s + XP], e = xr[s + 1], bb = br[s];
-            const float nh = xr[s - XP], wh = xr[s - 1];
-            const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-            const float nv = (r == 0) ? nh : from_up;
-            const float wv = (c == 0) ? wh : xprev;
-            const float v = relax_k<KIND>(bb, nv, so, wv, e, a, denom, inv);
-            if (active) xr[s] = v;
-            xprev = active ? v : xprev;
+            const int s0 = it * L - L * w;
+            float so[L], e[L], bb[L], nh[L], wh[L];
+#pragma unroll
+            for (int q = 0; q < L; ++q) {
+                const int sc = min(max(s0 + q, 0), STEPS - 1);
+                so[q] = xr[sc + XP];
+                e[q] = xr[sc + 1];
+                bb[q] = br[sc];
+                nh[q] = xr[sc - XP];
+                wh[q] = xr[sc - 1];
+            }
+#pragma unroll
+            for (int q = 0; q < L; ++q) {
+                const int s = s0 + q;
+                const int c = s - r;
+                const bool active = wave_ok && (unsigned)c < (unsigned)TW;
+                const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                const float nv = (r == 0) ? nh[q] : from_up;
+                const float wv = (c == 0) ? wh[q] : xprev;
+                const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, denom, inv);
+                if (active) xr[s] = v;
+                xprev = active ? v : xprev;
+            }
         }
     } else {
+        // Units that meet the ring keep it as data: a relaxed edge cell also
+        // writes its sign-adjusted ring cell, in LDS for the next sweep (which
+        // reads it two steps later, like any other neighbour) and to the grid
+        // at write-back for the next unit.  Sweep 0 reads the stored ring.
         const bool row_ok = wave_ok && i >= 1 && i <= n;
-        const bool top = (i == 1), bot = (i == n), kpos = (k > 0), zero0 = zero && k == 0;
-        for (int s = 0; s < GS_ROWS + TW - 1; ++s) {
+        const bool top = (i == 1), bot = (i == n);
+        const bool n_from_lds = (r == 0) || top;
+        for (int it = 0; it < NINT; ++it) {
             __syncthreads();
-            const int c = s - r;
-            const int j = jbase + c;
-            const bool active = row_ok && c >= 0 && c < TW && j >= 1 && j <= n;
-            const int lj = min(max(c + ofs, 1), XW - 2);
-            const int bj = min(max(c + ofs - 1, 0), BW - 1);
-            const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-            const float s_lds = xrow[lj + XP];
-            const float e_lds = xrow[lj + 1];
-            const float bb = brow[bj];
-            const bool left = (j == 1), right = (j == n);
-            float self_old = 0.f, n_lds = 0.f, w_lds = 0.f;
-            if (top || bot || left || right) self_old = xrow[lj];
-            if (r == 0 || top) n_lds = xrow[lj - XP];
-            if (c == 0 || left) w_lds = xrow[lj - 1];
-            const float tbv = sgn(rg.neg_tb, self_old), lrv = sgn(rg.neg_lr, self_old);
-            const float nv = top ? (kpos ? tbv : (zero ? 0.f : n_lds)) : (r == 0 ? n_lds : from_up);
-            const float wv = left ? (kpos ? lrv : (zero ? 0.f : w_lds)) : (c == 0 ? w_lds : xprev);
-            const float sv = bot ? (kpos ? tbv : (zero ? 0.f : s_lds)) : (zero0 ? 0.f : s_lds);
-            const float ev = right ? (kpos ? lrv : (zero ? 0.f : e_lds)) : (zero0 ? 0.f : e_lds);
-            const float v = relax_k<KIND>(bb, nv, sv, wv, ev, a, denom, inv);
-            if (active) xrow[lj] = v;
-            xprev = active ? v : xprev;
+            const int s0 = it * L - L * w;
+            float s_lds[L], e_lds[L], bb[L], n_lds[L], w_lds[L];
+#pragma unroll
+            for (int q = 0; q < L; ++q) {
+                const int c = s0 + q - r;
+                const int lj = min(max(c + ofs, 1), XW - 2);
+                const int bj = min(max(c + ofs - 1, 0), BW - 1);
+                s_lds[q] = xrow[lj + XP];
+                e_lds[q] = xrow[lj + 1];
+                bb[q] = brow[bj];
+                n_lds[q] = xrow[lj - XP];
+                w_lds[q] = xrow[lj - 1];
+            }
+#pragma unroll
+            for (int q = 0; q < L; ++q) {
+                const int c = s0 + q - r;
+                const int j = jbase + c;
+                const bool active = row_ok && (unsigned)c < (unsigned)TW && j >= 1 && j <= n;
+                const bool left = (j == 1), right = (j == n);
+                const int lj = min(max(c + ofs, 1), XW - 2);
+                const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                const float nv = n_from_lds ? n_lds[q] : from_up;
+                const float wv = (c == 0 || left) ? w_lds[q] : xprev;
+                const float v = relax_k<KIND>(bb[q], nv, s_lds[q], wv, e_lds[q], a, denom, inv);
+                if (active) {
+                    xrow[lj] = v;
+                    const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
+                    if (top) xrow[lj - XP] = tb;
+                    if (bot) xrow[lj + XP] = tb;
+                    if (left) xrow[lj - 1] = lr;
+                    if (right) xrow[lj + 1] = lr;
+                }
+                xprev = active ? v : xprev;
+            }
         }
     }
     __syncthreads();
 
-    // ---- write back every cell this unit relaxed (its latest sweep); cells
-    // relaxed at the solve's last sweep also write the ring they border.
+    // ---- write back every cell this unit relaxed (its latest sweep); edge
+    // cells also write the ring they border (after the solve's last sweep
+    // that is the final ring).
     for (int idx = tid; idx < XH * XW; idx += NT) {
         const int li = idx / XW, lj = idx % XW;
         const int gi = gi_lo + li, gj = gj_lo + lj;
@@ -384,7 +423,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
         const float v = X[li * XP + lj];
         const size_t o = (size_t)gi * P + gj;
         x[o] = v;
-        if (hi == K - 1) {
+        {  // the ring of the cell's latest sweep: what the next unit reading it expects
             const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
             const float cn = 0.5f * (tb + lr);
             if (gi == 1) x[o - P] = tb;
@@ -559,10 +598,11 @@ hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW) {
     return hipMemcpy(plan->work, work.data(), sizeof(int2) * acc, hipMemcpyHostToDevice);
 }
 
-hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M) {
+hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int lag) {
     const int n = R - 2;
     plan->TW = TW;
     plan->M = M;
+    plan->lag = lag;
     plan->GI = (n + K - 1 + GS_ROWS - 1) / GS_ROWS;
     plan->GJ = (n + K - 1 + TW - 1) / TW;
     plan->K = K;
@@ -608,12 +648,22 @@ void gs_plan_destroy(GsPlan* plan) {
     plan->host_offsets = nullptr;
 }
 
-template <int M>
+template <int M, int L>
 void launch_unit(hipStream_t s, int n, const int2* w, const SweepArgs& a) {
     if (a.kind == SWEEP_DIFFUSE)
-        hipLaunchKernelGGL((k_gs_unit<64, M, SWEEP_DIFFUSE>), dim3(n), dim3(64 * M), 0, s, a, w);
+        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_DIFFUSE>), dim3(n), dim3(64 * M), 0, s, a, w);
     else
-        hipLaunchKernelGGL((k_gs_unit<64, M, SWEEP_PRESSURE>), dim3(n), dim3(64 * M), 0, s, a, w);
+        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_PRESSURE>), dim3(n), dim3(64 * M), 0, s, a, w);
+}
+
+template <int M>
+bool launch_unit_lag(hipStream_t s, int lag, int n, const int2* w, const SweepArgs& a) {
+    switch (lag) {
+        case 1: launch_unit<M, 1>(s, n, w, a); return true;
+        case 4: launch_unit<M, 4>(s, n, w, a); return true;
+        case 8: launch_unit<M, 8>(s, n, w, a); return true;
+        default: return false;
+    }
 }
 
 hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a) {
@@ -636,12 +686,14 @@ hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a
                 return hipErrorInvalidValue;
         } else {
             if (plan.TW != 64) return hipErrorInvalidValue;
+            bool ok = false;
             switch (plan.M) {
-                case 2: launch_unit<2>(s, n, w, a); break;
-                case 4: launch_unit<4>(s, n, w, a); break;
-                case 8: launch_unit<8>(s, n, w, a); break;
-                default: return hipErrorInvalidValue;
+                case 2: ok = launch_unit_lag<2>(s, plan.lag, n, w, a); break;
+                case 4: ok = launch_unit_lag<4>(s, plan.lag, n, w, a); break;
+                case 8: ok = launch_unit_lag<8>(s, plan.lag, n, w, a); break;
+                default: break;
             }
+            if (!ok) return hipErrorInvalidValue;
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -29,7 +29,8 @@ class Fs2dError(RuntimeError):
 
 class Params(ctypes.Structure):
     _fields_ = [("diffuse_iterations", _i), ("pressure_iterations", _i), ("sweep_order", _i),
-                ("jacobi_block", _i), ("gs_tile_cols", _i), ("gs_block", _i), ("reserved", _i * 2)]
+                ("jacobi_block", _i), ("gs_tile_cols", _i), ("gs_block", _i), ("gs_lag", _i),
+                ("reserved", _i * 1)]
 
 
 _lib = None
@@ -93,7 +94,7 @@ class Solver:
 
     def __init__(self, rows: int, cols: int | None = None, diffuse_iterations: int = 20,
                  pressure_iterations: int = 20, order: int = GS_LEX, jacobi_block: int = 0,
-                 gs_tile_cols: int = 0, gs_block: int = 0):
+                 gs_tile_cols: int = 0, gs_block: int = 0, gs_lag: int = 0):
         L = library()
         cols = rows if cols is None else cols
         prm = Params()
@@ -104,6 +105,7 @@ class Solver:
         prm.jacobi_block = jacobi_block
         prm.gs_tile_cols = gs_tile_cols
         prm.gs_block = gs_block
+        prm.gs_lag = gs_lag
         h = _p()
         rc = L.fs2d_hip_create(rows, cols, ctypes.byref(prm), ctypes.byref(h))
         if rc != 0:

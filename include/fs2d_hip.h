@@ -60,7 +60,8 @@ typedef struct fs2d_params {
     int jacobi_block;        /* Jacobi sweeps fused per launch (temporal block depth); 0 = autotuned    */
     int gs_tile_cols;        /* Gauss-Seidel tile width in cells; 0 = default                            */
     int gs_block;            /* Gauss-Seidel sweeps per tile (temporal block): 1, 2, 4 or 8; 0 = default */
-    int reserved[2];         /* must be 0                                                                */
+    int gs_lag;              /* steps each sweep trails the previous one in a tile: 1, 4 or 8; 0 = default */
+    int reserved[1];         /* must be 0                                                                */
 } fs2d_params;
 
 typedef struct fs2d_handle fs2d_handle;

@@ -364,3 +364,235 @@ def gs_unit_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, TW=64)
             for (i, j), val in out.items():
                 x[i, j] = val
     return x
+
+
+def gs_unit_lag_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, L, TW=64):
+    """k_gs_unit with lagged waves: wave w runs its local step sigma at tick
+    sigma + L*w; a barrier opens every interval of L ticks, after which each
+    wave reads from LDS every operand of its L steps at once and then relaxes
+    them.  Asserts that no operand a wave prefetches in an interval is written
+    by another wave in the same interval, and replays the unit with the values
+    present at the interval's start."""
+    R = x.shape[0]
+    n = R - 2
+    a, denom = F(a), F(denom)
+    XH, XW = ROWS + 1 + M, TW + 1 + M
+    STEPS = ROWS + TW - 1
+    TICKS = STEPS + L * (M - 1)
+    for level in gs_unit_plan(R, K, TW, M):
+        snap = x.copy()
+        reads, writes, updates = [], [], []
+        for (I, J, kap) in level:
+            rd = set()
+            k_lo, k_hi, k_top = kap * M, min(kap * M + M, K) - 1, kap * M + M - 1
+            gi_lo, gj_lo = ROWS * I - k_top, TW * J - k_top
+            X = np.zeros((XH, XW), np.float32)
+            for li in range(XH):
+                for lj in range(XW):
+                    X[li, lj] = snap[min(max(gi_lo + li, 0), R - 1), min(max(gj_lo + lj, 0), R - 1)]
+            xprev = np.zeros((M, ROWS), np.float32)
+            for b0 in range(0, TICKS, L):
+                # prefetch: every wave captures its operands for ticks b0 .. b0+L-1
+                cap = {}
+                cap_cells = {w: set() for w in range(M)}
+                for w in range(M):
+                    k = kap * M + w
+                    for q in range(L):
+                        s = b0 + q - L * w
+                        for r in range(ROWS):
+                            c = s - r
+                            i, j = ROWS * I + 1 + r - k, TW * J + 1 + c - k
+                            if k >= K or not (0 <= c < TW and 1 <= i <= n and 1 <= j <= n):
+                                continue
+                            li, lj = i - gi_lo, j - gj_lo
+                            ops = {}
+                            for nm, (pi, pj) in (("self", (li, lj)), ("n", (li - 1, lj)), ("s", (li + 1, lj)),
+                                                 ("w", (li, lj - 1)), ("e", (li, lj + 1))):
+                                assert 0 <= pi < XH and 0 <= pj < XW
+                                ops[nm] = X[pi, pj]
+                                cap_cells[w].add((pi, pj))
+                                rd.add((gi_lo + pi, gj_lo + pj))
+                            cap[(w, q, r)] = ops
+                # relax
+                wrote = {w: set() for w in range(M)}
+                for q in range(L):
+                    for w in range(M):
+                        k = kap * M + w
+                        s = b0 + q - L * w
+                        from_up = np.concatenate([[F(0)], xprev[w, :-1]])
+                        new_prev = xprev[w].copy()
+                        for r in range(ROWS):
+                            if (w, q, r) not in cap:
+                                continue
+                            c = s - r
+                            i, j = ROWS * I + 1 + r - k, TW * J + 1 + c - k
+                            li, lj = i - gi_lo, j - gj_lo
+                            o = cap[(w, q, r)]
+                            zero0 = zero_start and k == 0
+                            so = o["self"]
+                            if i == 1:
+                                N = _sgn(neg_tb, so) if k > 0 else (F(0) if zero_start else o["n"])
+                            else:
+                                N = o["n"] if r == 0 else from_up[r]
+                            if j == 1:
+                                Wv = _sgn(neg_lr, so) if k > 0 else (F(0) if zero_start else o["w"])
+                            else:
+                                Wv = o["w"] if c == 0 else xprev[w, r]
+                            if i == n:
+                                S = _sgn(neg_tb, so) if k > 0 else (F(0) if zero_start else o["s"])
+                            else:
+                                S = F(0) if zero0 else o["s"]
+                            if j == n:
+                                E = _sgn(neg_lr, so) if k > 0 else (F(0) if zero_start else o["e"])
+                            else:
+                                E = F(0) if zero0 else o["e"]
+                            v = _relax(kind, b[i, j], N, S, Wv, E, a, denom)
+                            X[li, lj] = v
+                            wrote[w].add((li, lj))
+                            new_prev[r] = v
+                        xprev[w] = new_prev
+                for w in range(M):
+                    for w2 in range(M):
+                        if w != w2:
+                            assert not (cap_cells[w] & wrote[w2]), ("interval race", b0, w, w2)
+            out = {}
+            for li in range(XH):
+                for lj in range(XW):
+                    gi, gj = gi_lo + li, gj_lo + lj
+                    ok, khi = _unit_computed(gi, gj, I, J, k_lo, k_hi, n, TW)
+                    if not ok:
+                        continue
+                    out[(gi, gj)] = X[li, lj]
+                    if khi == K - 1:
+                        v = X[li, lj]
+                        tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                        cn = F(F(0.5) * F(tb + lr))
+                        if gi == 1: out[(0, gj)] = tb
+                        if gi == n: out[(n + 1, gj)] = tb
+                        if gj == 1: out[(gi, 0)] = lr
+                        if gj == n: out[(gi, n + 1)] = lr
+                        if gi == 1 and gj == 1: out[(0, 0)] = cn
+                        if gi == 1 and gj == n: out[(0, n + 1)] = cn
+                        if gi == n and gj == 1: out[(n + 1, 0)] = cn
+                        if gi == n and gj == n: out[(n + 1, n + 1)] = cn
+            reads.append(rd)
+            writes.append(set(out.keys()))
+            updates.append(out)
+        for t in range(len(level)):
+            for u in range(len(level)):
+                if t != u:
+                    assert not (reads[t] & writes[u]), ("read/write overlap", level[t], level[u])
+                    assert not (writes[t] & writes[u]), ("write/write overlap", level[t], level[u])
+        for out in updates:
+            for (i, j), val in out.items():
+                x[i, j] = val
+    return x
+
+
+def gs_unit_ring_model(x, b, K, kind, a, denom, neg_tb, neg_lr, M, L, TW=64):
+    """k_gs_unit, final form: lagged waves (as gs_unit_lag_model) with the ring
+    kept as data instead of logic.  Each relaxed edge cell also writes its
+    sign-adjusted ring cell -- in LDS for the next sweep of the same unit, and
+    to the grid at write-back for the next unit -- so every neighbour is read
+    the same way.  A zero-start solve (pressure) is run on an iterate that the
+    caller zero-filled, ring included."""
+    R = x.shape[0]
+    n = R - 2
+    a, denom = F(a), F(denom)
+    XH, XW = ROWS + 1 + M, TW + 1 + M
+    STEPS = ROWS + TW - 1
+    TICKS = STEPS + L * (M - 1)
+    for level in gs_unit_plan(R, K, TW, M):
+        snap = x.copy()
+        reads, writes, updates = [], [], []
+        for (I, J, kap) in level:
+            rd = set()
+            k_lo, k_hi, k_top = kap * M, min(kap * M + M, K) - 1, kap * M + M - 1
+            gi_lo, gj_lo = ROWS * I - k_top, TW * J - k_top
+            X = np.zeros((XH, XW), np.float32)
+            for li in range(XH):
+                for lj in range(XW):
+                    X[li, lj] = snap[min(max(gi_lo + li, 0), R - 1), min(max(gj_lo + lj, 0), R - 1)]
+            xprev = np.zeros((M, ROWS), np.float32)
+            for b0 in range(0, TICKS, L):
+                cap, cap_cells = {}, {w: set() for w in range(M)}
+                for w in range(M):
+                    k = kap * M + w
+                    for q in range(L):
+                        s = b0 + q - L * w
+                        for r in range(ROWS):
+                            c = s - r
+                            i, j = ROWS * I + 1 + r - k, TW * J + 1 + c - k
+                            if k >= K or not (0 <= c < TW and 1 <= i <= n and 1 <= j <= n):
+                                continue
+                            li, lj = i - gi_lo, j - gj_lo
+                            ops = {}
+                            for nm, (pi, pj) in (("n", (li - 1, lj)), ("s", (li + 1, lj)), ("w", (li, lj - 1)),
+                                                 ("e", (li, lj + 1))):
+                                assert 0 <= pi < XH and 0 <= pj < XW
+                                ops[nm] = X[pi, pj]
+                                cap_cells[w].add((pi, pj))
+                                rd.add((gi_lo + pi, gj_lo + pj))
+                            cap[(w, q, r)] = ops
+                wrote = {w: set() for w in range(M)}
+                for q in range(L):
+                    for w in range(M):
+                        k = kap * M + w
+                        s = b0 + q - L * w
+                        from_up = np.concatenate([[F(0)], xprev[w, :-1]])
+                        new_prev = xprev[w].copy()
+                        for r in range(ROWS):
+                            if (w, q, r) not in cap:
+                                continue
+                            c = s - r
+                            i, j = ROWS * I + 1 + r - k, TW * J + 1 + c - k
+                            li, lj = i - gi_lo, j - gj_lo
+                            o = cap[(w, q, r)]
+                            N = o["n"] if (r == 0 or i == 1) else from_up[r]
+                            Wv = o["w"] if (c == 0 or j == 1) else xprev[w, r]
+                            v = _relax(kind, b[i, j], N, o["s"], Wv, o["e"], a, denom)
+                            X[li, lj] = v
+                            wrote[w].add((li, lj))
+                            tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                            for cond, (pi, pj), val in ((i == 1, (li - 1, lj), tb), (i == n, (li + 1, lj), tb),
+                                                        (j == 1, (li, lj - 1), lr), (j == n, (li, lj + 1), lr)):
+                                if cond:
+                                    X[pi, pj] = val
+                                    wrote[w].add((pi, pj))
+                            new_prev[r] = v
+                        xprev[w] = new_prev
+                for w in range(M):
+                    for w2 in range(M):
+                        if w != w2:
+                            assert not (cap_cells[w] & wrote[w2]), ("interval race", b0, w, w2)
+            out = {}
+            for li in range(XH):
+                for lj in range(XW):
+                    gi, gj = gi_lo + li, gj_lo + lj
+                    ok, khi = _unit_computed(gi, gj, I, J, k_lo, k_hi, n, TW)
+                    if not ok:
+                        continue
+                    v = X[li, lj]
+                    out[(gi, gj)] = v
+                    tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                    cn = F(F(0.5) * F(tb + lr))
+                    if gi == 1: out[(0, gj)] = tb
+                    if gi == n: out[(n + 1, gj)] = tb
+                    if gj == 1: out[(gi, 0)] = lr
+                    if gj == n: out[(gi, n + 1)] = lr
+                    if gi == 1 and gj == 1: out[(0, 0)] = cn
+                    if gi == 1 and gj == n: out[(0, n + 1)] = cn
+                    if gi == n and gj == 1: out[(n + 1, 0)] = cn
+                    if gi == n and gj == n: out[(n + 1, n + 1)] = cn
+            reads.append(rd)
+            writes.append(set(out.keys()))
+            updates.append(out)
+        for t in range(len(level)):
+            for u in range(len(level)):
+                if t != u:
+                    assert not (reads[t] & writes[u]), ("read/write overlap", level[t], level[u])
+                    assert not (writes[t] & writes[u]), ("write/write overlap", level[t], level[u])
+        for out in updates:
+            for (i, j), val in out.items():
+                x[i, j] = val
+    return x

@@ -36,10 +36,10 @@ def assert_same_nan(got, want, what):
     assert bits_equal(g, w), f"{what}: {describe(g, w)}"
 
 
-def make(R, state, order=GS, kd=20, kp=20, block=0, gsb=0):
+def make(R, state, order=GS, kd=20, kp=20, block=0, gsb=0, lag=0):
     rho, u, v, rs, us, vs = state
     s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order, jacobi_block=block,
-                        gs_block=gsb)
+                        gs_block=gsb, gs_lag=lag)
     s.upload(rho, u, v)
     s.set_sources(rs, us, vs)
     return s
@@ -75,17 +75,18 @@ def test_add_sources(R, oracle_lib):
         assert_same(got[f], want, f"add_sources R={R} field={f}")
 
 
-SWEEP_VARIANTS = [(GS, 0, 1), (GS, 0, 2), (GS, 0, 4), (GS, 0, 8), (JAC, 1, 0), (JAC, 3, 0), (JAC, 4, 0),
-                  (JAC, 8, 0)]
+# (order, jacobi_block, gs_block, gs_lag)
+GS_VARIANTS = [(GS, 0, 1, 0), (GS, 0, 2, 1), (GS, 0, 2, 8), (GS, 0, 4, 4), (GS, 0, 8, 1), (GS, 0, 8, 8)]
+SWEEP_VARIANTS = GS_VARIANTS + [(JAC, 1, 0, 0), (JAC, 3, 0, 0), (JAC, 4, 0, 0), (JAC, 8, 0, 0)]
 
 
-@pytest.mark.parametrize("order,block,gsb", SWEEP_VARIANTS)
+@pytest.mark.parametrize("order,block,gsb,lag", SWEEP_VARIANTS)
 @pytest.mark.parametrize("R", [4, 10, 67, 130, 258])
 @pytest.mark.parametrize("rate", [0.0, 1e-3, 5e-2])
-def test_diffuse(R, rate, order, block, gsb, oracle_lib):
+def test_diffuse(R, rate, order, block, gsb, lag, oracle_lib):
     st = W.random_state(R, seed=11 * R)
     for K in (1, 2, 7, 20):
-        with make(R, st, order=order, block=block, gsb=gsb) as s:
+        with make(R, st, order=order, block=block, gsb=gsb, lag=lag) as s:
             for f in range(3):
                 s.diffuse(f, rate, 0.1, K)
             got = s.download()
@@ -93,7 +94,7 @@ def test_diffuse(R, rate, order, block, gsb, oracle_lib):
             want = st[f].copy()
             oracle_lib.diffuse(want, f, rate, 0.1, K, oracle_order(order))
             assert_same(got[f], want, f"diffuse R={R} field={f} rate={rate} K={K} order={order} "
-                                      f"block={block} gs_block={gsb}")
+                                      f"block={block} gs_block={gsb} lag={lag}")
 
 
 @pytest.mark.parametrize("R", SIZES)
@@ -150,19 +151,18 @@ def test_non_finite_velocity(oracle_lib):
         assert_same_nan(got[f], want[f], f"non-finite field={f}")
 
 
-@pytest.mark.parametrize("order,block,gsb", [(GS, 0, 1), (GS, 0, 2), (GS, 0, 4), (GS, 0, 8), (JAC, 2, 0),
-                                              (JAC, 4, 0), (JAC, 5, 0)])
+@pytest.mark.parametrize("order,block,gsb,lag", GS_VARIANTS + [(JAC, 2, 0, 0), (JAC, 4, 0, 0), (JAC, 5, 0, 0)])
 @pytest.mark.parametrize("R", [3, 4, 10, 67, 130, 258])
-def test_project(R, order, block, gsb, oracle_lib):
+def test_project(R, order, block, gsb, lag, oracle_lib):
     st = W.random_state(R, seed=7 * R, vel_scale=1.0)
     for K in (0, 1, 3, 20):
-        with make(R, st, order=order, block=block, gsb=gsb) as s:
+        with make(R, st, order=order, block=block, gsb=gsb, lag=lag) as s:
             s.project(K)
             _, gu, gv = s.download()
             gp = s.pressure()
         wu, wv = st[1].copy(), st[2].copy()
         wp = oracle_lib.project(wu, wv, K, oracle_order(order), want_p=True)
-        assert_same(gp, wp, f"pressure R={R} K={K} order={order} gs_block={gsb}")
+        assert_same(gp, wp, f"pressure R={R} K={K} order={order} gs_block={gsb} lag={lag}")
         assert_same(gu, wu, f"project u R={R} K={K} order={order}")
         assert_same(gv, wv, f"project v R={R} K={K} order={order}")
 
@@ -242,15 +242,15 @@ def test_config1_against_reference():
 
 
 # ------------------------------------------------ whole steps vs the oracle
-@pytest.mark.parametrize("order,block,gsb", [(GS, 0, 1), (GS, 0, 2), (GS, 0, 4), (GS, 0, 8), (JAC, 4, 0)])
+@pytest.mark.parametrize("order,block,gsb,lag", GS_VARIANTS + [(JAC, 4, 0, 0)])
 @pytest.mark.parametrize("R", [67, 130, 258])
-def test_steps_vs_oracle(R, order, block, gsb, oracle_lib):
+def test_steps_vs_oracle(R, order, block, gsb, lag, oracle_lib):
     st = W.vortex_state(R, seed=R, n_vortices=16)
     rs, us, vs = W.disc_sources(R, radius=R / 16)
     rho, u, v = (x.copy() for x in st)
     want = [x.copy() for x in st]
     with fs2d_amd.Solver(R, diffuse_iterations=12, pressure_iterations=30, order=order, jacobi_block=block,
-                         gs_block=gsb) as s:
+                         gs_block=gsb, gs_lag=lag) as s:
         s.upload(rho, u, v)
         s.set_sources(rs, us, vs)
         done = 0
@@ -287,15 +287,15 @@ def big_state():
     return W.vortex_state(R, seed=3)
 
 
-@pytest.mark.parametrize("gsb", [1, 4, 8])
-def test_full_size_stages(big_state, gsb, oracle_lib):
+@pytest.mark.parametrize("gsb,lag", [(1, 0), (4, 1), (8, 8)])
+def test_full_size_stages(big_state, gsb, lag, oracle_lib):
     """4096^2 interior: each stage against the oracle with short solves (the
     oracle cannot afford 80 sweeps here); exercises the full tile grid."""
     R = 4098
     rho, u, v = big_state
     z = np.zeros((R, R), np.float32)
     st = (rho, u, v, z, z, z)
-    with make(R, st, kd=3, kp=3, gsb=gsb) as s:
+    with make(R, st, kd=3, kp=3, gsb=gsb, lag=lag) as s:
         s.diffuse(0, 1e-5, 0.05, 3)
         s.advect_density(0.05)
         s.project(3)

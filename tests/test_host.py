@@ -15,7 +15,8 @@ import fs2d_amd
 import oracle
 from conftest import ROOT, bits_equal
 from fs2d_amd import workloads as W
-from schedule_models import gs_tile_model, gs_unit_model, jacobi_block_model
+from schedule_models import (gs_tile_model, gs_unit_lag_model, gs_unit_model, gs_unit_ring_model,
+                             jacobi_block_model)
 
 
 def header_symbols():
@@ -49,6 +50,7 @@ def test_default_params_are_the_reference_literals():
     (10, 10, {"jacobi_block": 9}, "jacobi_block"),
     (10, 10, {"gs_tile_cols": 48}, "gs_tile_cols"),
     (10, 10, {"gs_block": 3}, "gs_block"),
+    (10, 10, {"gs_lag": 2}, "gs_lag"),
 ])
 def test_create_validates_arguments_before_touching_the_device(rows, cols, kw, msg):
     with pytest.raises(fs2d_amd.Fs2dError, match=msg):
@@ -99,6 +101,49 @@ def test_gs_unit_schedule_reproduces_gauss_seidel(R, M, oracle_lib):
     p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
     p = np.full((R, R), 7.0, np.float32)
     gs_unit_model(p, d, 9, 1, 0, 0, False, False, 1, M)
+    assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M,L", [(3, 2, 1), (10, 4, 4), (67, 2, 8), (70, 8, 8)])
+def test_gs_unit_lagged_schedule_reproduces_gauss_seidel(R, M, L, oracle_lib):
+    """Lagged waves (sweep w trails sweep w-1 by L steps, one barrier per L
+    steps, operands of L steps read at once): bit-identical to GS-lex and no
+    wave writes, within an interval, a cell another wave read in it."""
+    rho, u, v, *_ = W.random_state(R, seed=R + 3)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        for K in (1, 9):
+            x = rho.copy()
+            gs_unit_lag_model(x, rho.copy(), K, 0, a, den, mode == 2, mode == 1, 0, M, L)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert bits_equal(x, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
+    p = np.full((R, R), 7.0, np.float32)
+    gs_unit_lag_model(p, d, 9, 1, 0, 0, False, False, 1, M, L)
+    assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M,L", [(3, 2, 1), (10, 4, 4), (67, 2, 8), (70, 8, 8), (70, 4, 4)])
+def test_gs_unit_ring_as_data_reproduces_gauss_seidel(R, M, L, oracle_lib):
+    """The kernel's final form: lagged waves, ring cells written as data by
+    the edge cells (in LDS and at write-back), zero-filled pressure start."""
+    rho, u, v, *_ = W.random_state(R, seed=R + 4)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        for K in (1, 9):
+            x = rho.copy()
+            gs_unit_ring_model(x, rho.copy(), K, 0, a, den, mode == 2, mode == 1, M, L)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert bits_equal(x, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
+    p = np.zeros((R, R), np.float32)
+    gs_unit_ring_model(p, d, 9, 1, 0, 0, False, False, M, L)
     assert bits_equal(p, p_ref)
 
 
