@@ -22,7 +22,7 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum { F_RHO, F_U, F_V, F_RS, F_US, F_VS, F_X0, F_T1, F_T2, F_D, F_P, F_SCRATCH, F_COUNT };
+enum { F_RHO, F_U, F_V, F_RS, F_US, F_VS, F_X0, F_T1, F_T2, F_D, F_P, F_SCRATCH, F_X0U, F_X0V, F_UN, F_VN, F_COUNT };
 
 }  // namespace
 
@@ -35,12 +35,14 @@ struct fs2d_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float* f[F_COUNT] = {};
     unsigned* max_bits = nullptr;
+    unsigned* speed_slots = nullptr;  // kSpeedSlots * kSlotStride words, zero between uses
     std::map<int, GsPlan> gs_plans;  // by sweep count
     int gs_tw = 64;
     int gs_block = 8;
-    int gs_lag = 8;
+    int gs_lag = 4;
     int jacobi_block = 4;
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
+    bool speed_valid = false;   // max_bits holds max |u|, |v| of the current velocity
     std::string err;
 
     float*& rho() { return f[F_RHO]; }
@@ -106,7 +108,8 @@ int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, f
         FS2D_HIP(h, gs_plan_for(h, K, &plan));
         if (plan->M > 0 && zero_start) {
             // the unit kernel starts from a zero-filled iterate, ring included
-            FS2D_HIP(h, launch_fill(h->stream, x, 0u, h->field_bytes / 4));
+            // (zero_start == 2: the caller has filled it already)
+            if (zero_start != 2) FS2D_HIP(h, launch_fill(h->stream, x, 0u, h->field_bytes / 4));
             A.zero_start = 0;
         }
         FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
@@ -124,20 +127,34 @@ int diffuse(fs2d_handle* h, float*& x, int mode, float rate, float dt, int K) {
     return linear_solve(h, x, h->f[F_X0], SWEEP_DIFFUSE, a, denom, mode, K, 0);
 }
 
+// Zero-start pressure iterate for the solver in use: the Gauss-Seidel unit
+// kernel and Jacobi take a zero-filled p; the one-sweep tile kernel reads
+// zeros itself.
+bool pressure_needs_fill(fs2d_handle* h) {
+    return !(h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block <= 1);
+}
+
 int project(fs2d_handle* h, int K) {
     const Scalars s = scalars(h->R, 1.0f);
-    FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h));
-    int rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1);
+    const bool fill = K > 0 && pressure_needs_fill(h);
+    FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h,
+                                  fill ? h->f[F_P] : nullptr));
+    int rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, fill ? 2 : 1);
     if (rc) return rc;
     h->pressure_zero = (K <= 0);
-    FS2D_HIP(h, launch_gradient(h->stream, h->u(), h->v(), h->f[F_P], h->R, h->P, s.h, h->pressure_zero ? 1 : 0));
+    FS2D_HIP(h, launch_gradient(h->stream, h->u(), h->v(), h->f[F_P], h->R, h->P, s.h, h->pressure_zero ? 1 : 0,
+                                h->max_bits, h->speed_slots));
+    h->speed_valid = true;
     return FS2D_OK;
 }
 
 int advect_density(fs2d_handle* h, float dt) {
     const Scalars s = scalars(h->R, dt);
-    FS2D_HIP(h, launch_fill(h->stream, h->max_bits, 0u, 1));
-    FS2D_HIP(h, launch_max_speed(h->stream, h->max_bits, h->u(), h->v(), h->R, h->P));
+    if (!h->speed_valid) {
+        FS2D_HIP(h, launch_fill(h->stream, h->max_bits, 0u, 1));
+        FS2D_HIP(h, launch_max_speed(h->stream, h->max_bits, h->u(), h->v(), h->R, h->P));
+        h->speed_valid = true;
+    }
     FS2D_HIP(h, launch_scatter_density(h->stream, h->f[F_T1], h->rho(), h->u(), h->v(), h->R, h->P, s.dt0,
                                        h->max_bits));
     std::swap(h->rho(), h->f[F_T1]);
@@ -146,10 +163,43 @@ int advect_density(fs2d_handle* h, float dt) {
 
 int advect_velocity(fs2d_handle* h, float dt) {
     const Scalars s = scalars(h->R, dt);
+    h->speed_valid = false;
     FS2D_HIP(h, launch_gather_velocity(h->stream, h->f[F_T1], h->f[F_T2], h->u(), h->v(), h->R, h->P, s.dt0));
     std::swap(h->u(), h->f[F_T1]);
     std::swap(h->v(), h->f[F_T2]);
     return FS2D_OK;
+}
+
+// The three diffuses of a step relaxed together (Gauss-Seidel unit plans):
+// density in place, u and v on fresh iterates so that the density scatter can
+// still read the pre-step velocity (src/fluid_solver_cpu.cpp:16-24).  Same
+// results as the sequential order: the three solves share no data.
+int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
+    const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
+    const int R = h->R, P = h->P;
+    int rc;
+    FS2D_HIP(h, launch_add_sources(h->stream, h->rho(), h->f[F_X0], h->f[F_RS], R, P, dt));
+    FS2D_HIP(h, launch_add_sources_split(h->stream, h->f[F_UN], h->f[F_X0U], h->u(), h->f[F_US], R, P, dt));
+    FS2D_HIP(h, launch_add_sources_split(h->stream, h->f[F_VN], h->f[F_X0V], h->v(), h->f[F_VS], R, P, dt));
+    if (Kd > 0) {
+        const float ad = dt * cells(R) * diff, av = dt * cells(R) * visc;
+        const float dd = 1.f + 4.f * ad, dv = 1.f + 4.f * av;
+        SweepSet set{};
+        set.count = 3;
+        set.a[0] = SweepArgs{h->rho(), h->f[F_X0], R, P, SWEEP_DIFFUSE, ad, dd, 1.0f / dd, ring_of(BND_CONT), Kd, 0};
+        set.a[1] = SweepArgs{h->f[F_UN], h->f[F_X0U], R, P, SWEEP_DIFFUSE, av, dv, 1.0f / dv, ring_of(BND_OPP_H), Kd, 0};
+        set.a[2] = SweepArgs{h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, dv, 1.0f / dv, ring_of(BND_OPP_V), Kd, 0};
+        const GsPlan* plan = nullptr;
+        FS2D_HIP(h, gs_plan_for(h, Kd, &plan));
+        FS2D_HIP(h, launch_gs_solve_set(h->stream, *plan, set));
+    }
+    if ((rc = advect_density(h, dt))) return rc;  // reads the pre-step u, v
+    std::swap(h->u(), h->f[F_UN]);
+    std::swap(h->v(), h->f[F_VN]);
+    h->speed_valid = false;
+    if ((rc = project(h, Kp))) return rc;
+    if ((rc = advect_velocity(h, dt))) return rc;
+    return project(h, Kp);
 }
 
 // fluid_solver_cpu::solve, src/fluid_solver_cpu.cpp:6-31.
@@ -157,12 +207,14 @@ int step(fs2d_handle* h, float diff, float visc, float dt) {
     const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
     const int R = h->R, P = h->P;
     int rc;
+    if (h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block > 1) return step_shared_diffuse(h, diff, visc, dt);
     // density: add source, diffuse, scatter-advect (:16-18)
     FS2D_HIP(h, launch_add_sources(h->stream, h->rho(), h->f[F_X0], h->f[F_RS], R, P, dt));
     if ((rc = diffuse(h, h->rho(), BND_CONT, diff, dt, Kd))) return rc;
     if ((rc = advect_density(h, dt))) return rc;
     // velocity: add sources, diffuse (:21-24; u and v are independent, so
     // each field's add is followed directly by its diffuse)
+    h->speed_valid = false;
     FS2D_HIP(h, launch_add_sources(h->stream, h->u(), h->f[F_X0], h->f[F_US], R, P, dt));
     if ((rc = diffuse(h, h->u(), BND_OPP_H, visc, dt, Kd))) return rc;
     FS2D_HIP(h, launch_add_sources(h->stream, h->v(), h->f[F_X0], h->f[F_VS], R, P, dt));
@@ -238,7 +290,7 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     h->prm = prm;
     h->gs_tw = prm.gs_tile_cols ? prm.gs_tile_cols : 64;
     h->gs_block = prm.gs_block ? prm.gs_block : 8;
-    h->gs_lag = prm.gs_lag ? prm.gs_lag : 8;
+    h->gs_lag = prm.gs_lag ? prm.gs_lag : 4;
     h->jacobi_block = prm.jacobi_block ? prm.jacobi_block : 4;
     auto bail = [&](hipError_t e, const char* what) {
         std::string msg = std::string("fs2d_hip_create: ") + what + ": " + hipGetErrorString(e);
@@ -257,6 +309,10 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
             return bail(e, "zero fill");
     }
     if ((e = hipMalloc(&h->max_bits, 256)) != hipSuccess) return bail(e, "hipMalloc(scalar)");
+    if ((e = hipMalloc(&h->speed_slots, sizeof(unsigned) * kSpeedSlots * kSlotStride)) != hipSuccess)
+        return bail(e, "hipMalloc(speed slots)");
+    if ((e = launch_fill(h->stream, h->speed_slots, 0u, kSpeedSlots * kSlotStride)) != hipSuccess)
+        return bail(e, "zero fill");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
     *out = h;
     return FS2D_OK;
@@ -269,6 +325,7 @@ int fs2d_hip_destroy(fs2d_handle* h) {
     for (float* p : h->f)
         if (p) (void)hipFree(p);
     if (h->max_bits) (void)hipFree(h->max_bits);
+    if (h->speed_slots) (void)hipFree(h->speed_slots);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -283,6 +340,7 @@ void* fs2d_hip_stream(fs2d_handle* h) { return h ? static_cast<void*>(h->stream)
 int fs2d_hip_upload(fs2d_handle* h, const float* rho, const float* u, const float* v) {
     if (!h || !rho || !u || !v) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_upload: NULL argument");
     int rc;
+    h->speed_valid = false;
     if ((rc = copy_in(h, h->rho(), rho, hipMemcpyHostToDevice))) return rc;
     if ((rc = copy_in(h, h->u(), u, hipMemcpyHostToDevice))) return rc;
     if ((rc = copy_in(h, h->v(), v, hipMemcpyHostToDevice))) return rc;
@@ -293,6 +351,7 @@ int fs2d_hip_upload(fs2d_handle* h, const float* rho, const float* u, const floa
 int fs2d_hip_upload_device(fs2d_handle* h, const float* rho, const float* u, const float* v) {
     if (!h || !rho || !u || !v) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_upload_device: NULL argument");
     int rc;
+    h->speed_valid = false;
     if ((rc = copy_in(h, h->rho(), rho, hipMemcpyDeviceToDevice))) return rc;
     if ((rc = copy_in(h, h->u(), u, hipMemcpyDeviceToDevice))) return rc;
     return copy_in(h, h->v(), v, hipMemcpyDeviceToDevice);
@@ -379,6 +438,7 @@ static float*& field_ref(fs2d_handle* h, int field) {
 int fs2d_hip_stage_add_sources(fs2d_handle* h, int field, float dt) {
     if (!h || !field_ok(field)) return fail(h, FS2D_ERR_ARGUMENT, "stage_add_sources: bad argument");
     const int src = field == FS2D_FIELD_DENSITY ? F_RS : (field == FS2D_FIELD_VELOCITY_U ? F_US : F_VS);
+    if (field != FS2D_FIELD_DENSITY) h->speed_valid = false;
     FS2D_HIP(h, launch_add_sources(h->stream, field_ref(h, field), nullptr, h->f[src], h->R, h->P, dt));
     return FS2D_OK;
 }
@@ -387,6 +447,7 @@ int fs2d_hip_stage_diffuse(fs2d_handle* h, int field, float rate, float dt, int 
     if (!h || !field_ok(field) || iterations < 0)
         return fail(h, FS2D_ERR_ARGUMENT, "stage_diffuse: bad argument");
     float*& x = field_ref(h, field);
+    if (field != FS2D_FIELD_DENSITY) h->speed_valid = false;
     FS2D_HIP(h, hipMemcpyAsync(h->f[F_X0], x, h->field_bytes, hipMemcpyDeviceToDevice, h->stream));
     return diffuse(h, x, field, rate, dt, iterations);
 }
@@ -408,6 +469,7 @@ int fs2d_hip_stage_project(fs2d_handle* h, int iterations) {
 
 int fs2d_hip_stage_set_boundary(fs2d_handle* h, int field) {
     if (!h || !field_ok(field)) return fail(h, FS2D_ERR_ARGUMENT, "stage_set_boundary: bad argument");
+    if (field != FS2D_FIELD_DENSITY) h->speed_valid = false;
     FS2D_HIP(h, launch_set_boundary(h->stream, field_ref(h, field), h->R, h->P, ring_of(field)));
     return FS2D_OK;
 }
@@ -417,7 +479,7 @@ int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
     if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_pressure_sweeps: bad argument");
     const Scalars s = scalars(h->R, 1.0f);
     const int K = h->prm.pressure_iterations;
-    FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h));
+    FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h, nullptr));
     int rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1);  // warm
     if (rc) return rc;
     FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
@@ -443,8 +505,9 @@ int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
     };
     const int d = solve_launches(Kd), p = solve_launches(Kp);
     if (d < 0 || p < 0) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
-    // add x3, diffuse x3, fill + max + scatter, project x2 (div + solve + grad), gather
-    *launches = 3 + 3 * d + 3 + 2 * (2 + p) + 1;
+    // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather
+    const bool shared = h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block > 1;
+    *launches = 3 + (shared ? d : 3 * d) + 1 + 2 * (2 + p) + 1;
     return FS2D_OK;
 }
 

@@ -46,17 +46,37 @@ struct SweepArgs {
     int zero_start;  // iterate starts at 0 everywhere, ring included (pressure, :186)
 };
 
+// Up to three independent solves with the same sweep count, relaxed by the
+// same level launches (e.g. the density, u and v diffuses of one step).
+struct SweepSet {
+    static constexpr int kMax = 3;
+    SweepArgs a[kMax];
+    int count;
+    int units_per_field;  // set per launch
+};
+
 // ---- launchers (stream-ordered, no synchronisation) ---------------------
 hipError_t launch_add_sources(hipStream_t s, float* x, float* x0, const float* src, int R, int P, float dt);
+// out = x + dt * src and x0 = out on the interior, out's ring = x's ring; x is
+// left untouched (the density scatter still needs the pre-step velocity).
+hipError_t launch_add_sources_split(hipStream_t s, float* out, float* x0, const float* x, const float* src, int R,
+                                    int P, float dt);
 hipError_t launch_max_speed(hipStream_t s, unsigned* out_bits, const float* u, const float* v, int R, int P);
 hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, const float* u,
                                   const float* v, int R, int P, float dt0, const unsigned* max_bits);
 hipError_t launch_gather_velocity(hipStream_t s, float* u1, float* v1, const float* u0, const float* v0,
                                   int R, int P, float dt0);
+// p_zero (may be null): also zero this pressure iterate, ring included.
 hipError_t launch_divergence(hipStream_t s, float* d, const float* u, const float* v, int R, int P,
-                             float neg_half_h);
+                             float neg_half_h, float* p_zero);
+// Max-speed reduction slots: kSpeedSlots words, kSlotStride words apart
+// (separate cache lines), zero between uses.
+constexpr int kSpeedSlots = 256, kSlotStride = 32;
+// max_bits (may be null): set it to the max finite |u|, |v| of the result,
+// through speed_slots (kSpeedSlots * kSlotStride words, all zero on entry,
+// left zero).
 hipError_t launch_gradient(hipStream_t s, float* u, float* v, const float* p, int R, int P, float h,
-                           int p_is_zero);
+                           int p_is_zero, unsigned* max_bits, unsigned* speed_slots);
 hipError_t launch_set_boundary(hipStream_t s, float* x, int R, int P, Ring ring);
 hipError_t launch_fill(hipStream_t s, void* p, uint32_t value, size_t n_words);
 
@@ -75,6 +95,8 @@ hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW);
 hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int lag);
 void gs_plan_destroy(GsPlan* plan);
 hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a);
+// All solves of the set in shared level launches (unit plans; a tile plan takes one).
+hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set);
 
 // Jacobi sweeps, `block` sweeps fused per launch; ping-pongs between x and
 // scratch and leaves the result in x.

@@ -54,6 +54,30 @@ __global__ __launch_bounds__(BX* BY) void k_add_sources(float* __restrict__ x, f
     if (x0) x0[o] = v;
 }
 
+// The same update written to a fresh iterate: out = x + dt*s (interior),
+// x0 = out, and out's ring copied from x (the stale ring the first sweep reads).
+__global__ __launch_bounds__(BX* BY) void k_add_sources_split(float* __restrict__ out, float* __restrict__ x0,
+                                                              const float* __restrict__ x, const float* __restrict__ s,
+                                                              int R, int P, float dt) {
+    int i, j;
+    if (!interior(R, i, j)) return;
+    const size_t o = (size_t)i * P + j;
+    const float v = x[o] + dt * s[o];
+    out[o] = v;
+    x0[o] = v;
+    if (i == 1) out[o - P] = x[o - P];
+    if (i == R - 2) out[o + P] = x[o + P];
+    if (j == 1) out[o - 1] = x[o - 1];
+    if (j == R - 2) out[o + 1] = x[o + 1];
+    if (i == 1 && j == 1) {
+        const size_t last = (size_t)(R - 1) * P;
+        out[0] = x[0];
+        out[R - 1] = x[R - 1];
+        out[last] = x[last];
+        out[last + R - 1] = x[last + R - 1];
+    }
+}
+
 // Largest finite |u|, |v| over the interior, as float bits (non-negative
 // floats order like their bit patterns).  Bounds the scatter's search window.
 __global__ __launch_bounds__(256) void k_max_speed(unsigned* __restrict__ out, const float* __restrict__ u,
@@ -88,11 +112,14 @@ __global__ __launch_bounds__(BX* BY) void k_scatter_density(float* __restrict__ 
                                                             int R, int P, float dt0, const unsigned* __restrict__ max_bits) {
     int di, dj;
     if (!interior(R, di, dj)) return;
+    // A source at row si lands on rows trunc(y), trunc(y) + 1 with
+    // |y - si| <= |fl(dt0 v)| + ulp(y) / 2 <= dt0 * max|v| + 2^-8 (y < 2^15),
+    // so it reaches row di only if |si - di| <= W = floor(that bound) + 1.
     const float mv = __uint_as_float(*max_bits);
-    const float reach = dt0 * mv;
-    const int r = (reach < (float)R) ? (int)reach + 2 : R;
-    const int i_lo = max(1, di - r - 1), i_hi = min(R - 2, di + r);
-    const int j_lo = max(1, dj - r - 1), j_hi = min(R - 2, dj + r);
+    const float reach = dt0 * mv + 0x1p-8f;
+    const int r = (reach < (float)R) ? (int)reach + 1 : R;
+    const int i_lo = max(1, di - r), i_hi = min(R - 2, di + r);
+    const int j_lo = max(1, dj - r), j_hi = min(R - 2, dj + r);
     const float hi = (float)R - 1.5f;
     float acc = 0.f;
     for (int si = i_lo; si <= i_hi; ++si) {
@@ -153,36 +180,88 @@ __global__ __launch_bounds__(BX* BY) void k_gather_velocity(float* __restrict__ 
 }
 
 // src/fluid_solver_cpu.cpp:188-193: d = (-0.5f * h) * (((uE - uW) + vS) - vN).
-// The divergence ring (:194) is never read and is not written.
+// The divergence ring (:194) is never read and is not written.  When p is
+// given, the pressure iterate is zeroed too, ring included (:186).
 __global__ __launch_bounds__(BX* BY) void k_divergence(float* __restrict__ d, const float* __restrict__ u,
-                                                       const float* __restrict__ v, int R, int P, float neg_half_h) {
+                                                       const float* __restrict__ v, int R, int P, float neg_half_h,
+                                                       float* __restrict__ p) {
     int i, j;
     if (!interior(R, i, j)) return;
     const size_t o = (size_t)i * P + j;
     d[o] = neg_half_h * (u[o + 1] - u[o - 1] + v[o + P] - v[o - P]);
+    if (p) {
+        p[o] = 0.f;
+        if (i == 1) p[o - P] = 0.f;
+        if (i == R - 2) p[o + P] = 0.f;
+        if (j == 1) p[o - 1] = 0.f;
+        if (j == R - 2) p[o + 1] = 0.f;
+        if (i == 1 && j == 1) {
+            p[0] = 0.f;
+            p[R - 1] = 0.f;
+            p[(size_t)(R - 1) * P] = 0.f;
+            p[(size_t)(R - 1) * P + R - 1] = 0.f;
+        }
+    }
 }
 
 // src/fluid_solver_cpu.cpp:206-214: u -= (0.5f * (pE - pW)) / h,
 // v -= (0.5f * (pS - pN)) / h, then the opposite rings of u and v.
 __global__ __launch_bounds__(BX* BY) void k_gradient(float* __restrict__ u, float* __restrict__ v,
                                                      const float* __restrict__ p, int R, int P, float h,
-                                                     int p_is_zero) {
+                                                     int p_is_zero, unsigned* __restrict__ speed_slots) {
     int i, j;
-    if (!interior(R, i, j)) return;
-    const size_t o = (size_t)i * P + j;
-    float pe = 0.f, pw = 0.f, ps = 0.f, pn = 0.f;
-    if (!p_is_zero) {
-        pe = p[o + 1];
-        pw = p[o - 1];
-        ps = p[o + P];
-        pn = p[o - P];
+    const bool in = interior(R, i, j);
+    float m = 0.f;
+    if (in) {
+        const size_t o = (size_t)i * P + j;
+        float pe = 0.f, pw = 0.f, ps = 0.f, pn = 0.f;
+        if (!p_is_zero) {
+            pe = p[o + 1];
+            pw = p[o - 1];
+            ps = p[o + P];
+            pn = p[o - P];
+        }
+        const float un = u[o] - 0.5f * (pe - pw) / h;
+        const float vn = v[o] - 0.5f * (ps - pn) / h;
+        u[o] = un;
+        v[o] = vn;
+        write_ring(u, i, j, un, R, P, Ring{false, true});
+        write_ring(v, i, j, vn, R, P, Ring{true, false});
+        if (fabsf(un) <= FLT_MAX) m = fabsf(un);  // finite values only
+        if (fabsf(vn) <= FLT_MAX) m = fmaxf(m, fabsf(vn));
     }
-    const float un = u[o] - 0.5f * (pe - pw) / h;
-    const float vn = v[o] - 0.5f * (ps - pn) / h;
-    u[o] = un;
-    v[o] = vn;
-    write_ring(u, i, j, un, R, P, Ring{false, true});
-    write_ring(v, i, j, vn, R, P, Ring{true, false});
+    if (speed_slots) {
+        // largest finite |u|, |v| of the result, for the next scatter: wave,
+        // then block reduction, one atomic per block spread over kSpeedSlots
+        // slots on separate cache lines (k_fold_speed folds them)
+        __shared__ float wmax[BX * BY / 64];
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        const int t = threadIdx.y * BX + threadIdx.x;
+        if ((t & 63) == 0) wmax[t >> 6] = m;
+        __syncthreads();
+        if (t == 0) {
+            for (int q = 1; q < BX * BY / 64; ++q) m = fmaxf(m, wmax[q]);
+            const unsigned slot = (blockIdx.y * gridDim.x + blockIdx.x) % kSpeedSlots;
+            atomicMax(speed_slots + slot * kSlotStride, __float_as_uint(m));
+        }
+    }
+}
+
+// Fold the per-slot maxima into the scalar the scatter reads, and clear the
+// slots for the next gradient.
+__global__ __launch_bounds__(kSpeedSlots) void k_fold_speed(unsigned* __restrict__ max_bits,
+                                                             unsigned* __restrict__ speed_slots) {
+    __shared__ unsigned part[kSpeedSlots / 64];
+    const int t = threadIdx.x;
+    unsigned m = speed_slots[t * kSlotStride];
+    speed_slots[t * kSlotStride] = 0u;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+    if ((t & 63) == 0) part[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+        for (int q = 1; q < kSpeedSlots / 64; ++q) m = max(m, part[q]);
+        *max_bits = m;
+    }
 }
 
 // Standalone ring (src/fluid_solver_cpu.cpp:33-83); every ring cell is a
@@ -216,6 +295,12 @@ hipError_t launch_add_sources(hipStream_t s, float* x, float* x0, const float* s
     return hipGetLastError();
 }
 
+hipError_t launch_add_sources_split(hipStream_t s, float* out, float* x0, const float* x, const float* src, int R,
+                                    int P, float dt) {
+    hipLaunchKernelGGL(k_add_sources_split, grid2d(R), dim3(BX, BY), 0, s, out, x0, x, src, R, P, dt);
+    return hipGetLastError();
+}
+
 hipError_t launch_max_speed(hipStream_t s, unsigned* out_bits, const float* u, const float* v, int R, int P) {
     hipLaunchKernelGGL(k_max_speed, dim3(std::min(R - 2, 2048)), dim3(256), 0, s, out_bits, u, v, R, P);
     return hipGetLastError();
@@ -234,14 +319,16 @@ hipError_t launch_gather_velocity(hipStream_t s, float* u1, float* v1, const flo
 }
 
 hipError_t launch_divergence(hipStream_t s, float* d, const float* u, const float* v, int R, int P,
-                             float neg_half_h) {
-    hipLaunchKernelGGL(k_divergence, grid2d(R), dim3(BX, BY), 0, s, d, u, v, R, P, neg_half_h);
+                             float neg_half_h, float* p_zero) {
+    hipLaunchKernelGGL(k_divergence, grid2d(R), dim3(BX, BY), 0, s, d, u, v, R, P, neg_half_h, p_zero);
     return hipGetLastError();
 }
 
 hipError_t launch_gradient(hipStream_t s, float* u, float* v, const float* p, int R, int P, float h,
-                           int p_is_zero) {
-    hipLaunchKernelGGL(k_gradient, grid2d(R), dim3(BX, BY), 0, s, u, v, p, R, P, h, p_is_zero);
+                           int p_is_zero, unsigned* max_bits, unsigned* speed_slots) {
+    hipLaunchKernelGGL(k_gradient, grid2d(R), dim3(BX, BY), 0, s, u, v, p, R, P, h, p_is_zero,
+                       max_bits ? speed_slots : nullptr);
+    if (max_bits) hipLaunchKernelGGL(k_fold_speed, dim3(1), dim3(kSpeedSlots), 0, s, max_bits, speed_slots);
     return hipGetLastError();
 }
 

@@ -257,7 +257,10 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
 // with one barrier per step.  Units with equal I' + J' + kappa touch disjoint
 // cells and are launched together.
 template <int TW, int M, int L, int KIND>
-__global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __restrict__ work) {
+__global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __restrict__ work) {
+    // blocks [f * units, (f + 1) * units) relax field f of the set, unit by unit
+    const int f = blockIdx.x / S.units_per_field;
+    const SweepArgs& A = S.a[f];
     constexpr int NT = 64 * M;
     constexpr int XH = GS_ROWS + 1 + M, XW = TW + 1 + M;            // iterate region
     constexpr int XP = XW + (XW & 1);                                // even pitch: skew is conflict-free
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
     __shared__ float X[XH * XP + GUARD];
     __shared__ float Bs[BH * BP + GUARD];
 
-    const int2 wk = work[blockIdx.x];
+    const int2 wk = work[blockIdx.x - f * S.units_per_field];
     const int I = wk.x >> 16, J = wk.x & 0xffff, kap = wk.y;
     const int R = A.R, P = A.P, n = R - 2, K = A.K;
     const int k_lo = kap * M, k_hi = min(k_lo + M, K) - 1, k_top = k_lo + M - 1;
@@ -340,6 +343,8 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
+            // a wave with no step of this interval in [0, STEPS) only joins the barrier
+            if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
             float so[L], e[L], bb[L], nh[L], wh[L];
 #pragma unroll
             for (int q = 0; q < L; ++q) {
@@ -374,6 +379,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepArgs A, const int2* __r
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
+            if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
             float s_lds[L], e_lds[L], bb[L], n_lds[L], w_lds[L];
 #pragma unroll
             for (int q = 0; q < L; ++q) {
@@ -649,25 +655,33 @@ void gs_plan_destroy(GsPlan* plan) {
 }
 
 template <int M, int L>
-void launch_unit(hipStream_t s, int n, const int2* w, const SweepArgs& a) {
-    if (a.kind == SWEEP_DIFFUSE)
-        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_DIFFUSE>), dim3(n), dim3(64 * M), 0, s, a, w);
+void launch_unit(hipStream_t s, int n, const int2* w, const SweepSet& set) {
+    SweepSet level = set;
+    level.units_per_field = n;
+    const dim3 grid(n * set.count);
+    if (set.a[0].kind == SWEEP_DIFFUSE)
+        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_DIFFUSE>), grid, dim3(64 * M), 0, s, level, w);
     else
-        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_PRESSURE>), dim3(n), dim3(64 * M), 0, s, a, w);
+        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_PRESSURE>), grid, dim3(64 * M), 0, s, level, w);
 }
 
 template <int M>
-bool launch_unit_lag(hipStream_t s, int lag, int n, const int2* w, const SweepArgs& a) {
+bool launch_unit_lag(hipStream_t s, int lag, int n, const int2* w, const SweepSet& set) {
     switch (lag) {
-        case 1: launch_unit<M, 1>(s, n, w, a); return true;
-        case 4: launch_unit<M, 4>(s, n, w, a); return true;
-        case 8: launch_unit<M, 8>(s, n, w, a); return true;
+        case 1: launch_unit<M, 1>(s, n, w, set); return true;
+        case 4: launch_unit<M, 4>(s, n, w, set); return true;
+        case 8: launch_unit<M, 8>(s, n, w, set); return true;
         default: return false;
     }
 }
 
-hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a) {
-    if (plan.K != a.K) return hipErrorInvalidValue;
+hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set) {
+    if (set.count < 1 || set.count > SweepSet::kMax) return hipErrorInvalidValue;
+    for (int f = 0; f < set.count; ++f)
+        if (plan.K != set.a[f].K || set.a[f].kind != set.a[0].kind || set.a[f].R != set.a[0].R)
+            return hipErrorInvalidValue;
+    if (plan.M == 0 && set.count != 1) return hipErrorInvalidValue;
+    const SweepArgs& a = set.a[0];
     for (int L = 0; L < plan.levels; ++L) {
         const int n = plan.host_offsets[L + 1] - plan.host_offsets[L];
         if (n <= 0) continue;
@@ -688,9 +702,9 @@ hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a
             if (plan.TW != 64) return hipErrorInvalidValue;
             bool ok = false;
             switch (plan.M) {
-                case 2: ok = launch_unit_lag<2>(s, plan.lag, n, w, a); break;
-                case 4: ok = launch_unit_lag<4>(s, plan.lag, n, w, a); break;
-                case 8: ok = launch_unit_lag<8>(s, plan.lag, n, w, a); break;
+                case 2: ok = launch_unit_lag<2>(s, plan.lag, n, w, set); break;
+                case 4: ok = launch_unit_lag<4>(s, plan.lag, n, w, set); break;
+                case 8: ok = launch_unit_lag<8>(s, plan.lag, n, w, set); break;
                 default: break;
             }
             if (!ok) return hipErrorInvalidValue;
@@ -699,6 +713,13 @@ hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a) {
+    SweepSet set{};
+    set.a[0] = a;
+    set.count = 1;
+    return launch_gs_solve_set(s, plan, set);
 }
 
 // ---- Jacobi ------------------------------------------------------------------
