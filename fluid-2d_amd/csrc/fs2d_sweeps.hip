@@ -267,8 +267,12 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     constexpr int BH = GS_ROWS - 1 + M, BW = TW - 1 + M;             // right-hand side region
     constexpr int BP = BW + (BW & 1);
     constexpr int NX = (XH * XW + NT - 1) / NT, NBX = (BH * BW + NT - 1) / NT;
-    constexpr int GUARD = GS_ROWS + TW + M;  // reads of inactive lanes in the plain sweep
-    __shared__ float X[XH * XP + GUARD];
+    // Guard space after each array absorbs the operand reads of inactive lanes
+    // (their skewed columns run up to GS_ROWS + TW + M past the cell); the
+    // DUMMY slots, one per thread, take the stores of lanes that store nothing.
+    constexpr int GUARD = GS_ROWS + TW + 2 * M + 8;
+    constexpr int DUMMY0 = XH * XP + GUARD;
+    __shared__ float X[XH * XP + GUARD + NT];
     __shared__ float Bs[BH * BP + GUARD];
 
     const int2 wk = work[blockIdx.x - f * S.units_per_field];
@@ -318,8 +322,6 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     const bool wave_ok = (k <= k_hi);
     const int jbase = TW * J + 1 - k;
     const int ofs = M - w;  // X column of skewed column 0 is 1 + (M - 1 - w)
-    float* const xrow = X + (1 + r + (M - 1 - w)) * XP;
-    const float* const brow = Bs + (r + (M - 1 - w)) * BP;
     const float a = A.a, denom = A.denom, inv = A.inv_denom;
     float xprev = 0.f;
     // A unit whose cells all lie in rows and columns 2..n-1 never meets the
@@ -337,9 +339,13 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     // with only the DPP / register chain between steps.
     constexpr int STEPS = GS_ROWS + TW - 1;
     constexpr int NINT = (STEPS + L * (M - 1) + L - 1) / L;
+    // The tick is branch-free: a lane that relaxes nothing this tick, and a
+    // lane whose cell borders no ring, stores to its own dummy LDS slot, so
+    // the batched operand reads cannot be sunk into per-tick branches.
+    const int me0 = (1 + r + (M - 1 - w)) * XP + ofs - r;  // X index of this lane's cell at step s: me0 + s
+    const int bme0 = (r + (M - 1 - w)) * BP + ofs - 1 - r;  // Bs index of its right-hand side: bme0 + s
+    const int dummy = DUMMY0 + tid;
     if (plain) {
-        float* const xr = xrow + ofs - r;           // xr[s] = cell at skewed column s - r
-        const float* const br = brow + ofs - 1 - r;
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
@@ -348,23 +354,24 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
             float so[L], e[L], bb[L], nh[L], wh[L];
 #pragma unroll
             for (int q = 0; q < L; ++q) {
-                const int sc = min(max(s0 + q, 0), STEPS - 1);
-                so[q] = xr[sc + XP];
-                e[q] = xr[sc + 1];
-                bb[q] = br[sc];
-                nh[q] = xr[sc - XP];
-                wh[q] = xr[sc - 1];
+                const int x = me0 + s0 + q;
+                so[q] = X[x + XP];
+                e[q] = X[x + 1];
+                bb[q] = Bs[bme0 + s0 + q];
+                nh[q] = X[x - XP];
+                wh[q] = X[x - 1];
             }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < L; ++q) {
                 const int s = s0 + q;
                 const int c = s - r;
-                const bool active = wave_ok && (unsigned)c < (unsigned)TW;
+                const bool active = (unsigned)c < (unsigned)TW;
                 const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
                 const float nv = (r == 0) ? nh[q] : from_up;
                 const float wv = (c == 0) ? wh[q] : xprev;
                 const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, denom, inv);
-                if (active) xr[s] = v;
+                X[active ? me0 + s : dummy] = v;
                 xprev = active ? v : xprev;
             }
         }
@@ -376,40 +383,43 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
         const bool row_ok = wave_ok && i >= 1 && i <= n;
         const bool top = (i == 1), bot = (i == n);
         const bool n_from_lds = (r == 0) || top;
+        const int c_left = 1 - jbase, c_right = n - jbase;  // skewed columns of j = 1 and j = n
+        const int c_lo = max(0, c_left), c_hi = min(TW - 1, c_right);
+        // ring rows: one store per tick serves top or bottom (both only when n == 1)
+        const int tb_ofs = top ? -XP : XP;
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
-            if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
+            if (!row_ok || s0 + L <= 0 || s0 >= STEPS) continue;
             float s_lds[L], e_lds[L], bb[L], n_lds[L], w_lds[L];
 #pragma unroll
             for (int q = 0; q < L; ++q) {
-                const int c = s0 + q - r;
-                const int lj = min(max(c + ofs, 1), XW - 2);
-                const int bj = min(max(c + ofs - 1, 0), BW - 1);
-                s_lds[q] = xrow[lj + XP];
-                e_lds[q] = xrow[lj + 1];
-                bb[q] = brow[bj];
-                n_lds[q] = xrow[lj - XP];
-                w_lds[q] = xrow[lj - 1];
+                const int x = me0 + s0 + q;
+                s_lds[q] = X[x + XP];
+                e_lds[q] = X[x + 1];
+                bb[q] = Bs[bme0 + s0 + q];
+                n_lds[q] = X[x - XP];
+                w_lds[q] = X[x - 1];
             }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < L; ++q) {
-                const int c = s0 + q - r;
-                const int j = jbase + c;
-                const bool active = row_ok && (unsigned)c < (unsigned)TW && j >= 1 && j <= n;
-                const bool left = (j == 1), right = (j == n);
-                const int lj = min(max(c + ofs, 1), XW - 2);
+                const int s = s0 + q;
+                const int c = s - r;
+                const bool active = c >= c_lo && c <= c_hi;
+                const bool left = (c == c_left), right = (c == c_right);
                 const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
                 const float nv = n_from_lds ? n_lds[q] : from_up;
                 const float wv = (c == 0 || left) ? w_lds[q] : xprev;
                 const float v = relax_k<KIND>(bb[q], nv, s_lds[q], wv, e_lds[q], a, denom, inv);
-                if (active) {
-                    xrow[lj] = v;
-                    const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
-                    if (top) xrow[lj - XP] = tb;
-                    if (bot) xrow[lj + XP] = tb;
-                    if (left) xrow[lj - 1] = lr;
-                    if (right) xrow[lj + 1] = lr;
+                const int x = me0 + s;
+                const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
+                X[active ? x : dummy] = v;
+                X[(active && (top || bot)) ? x + tb_ofs : dummy] = tb;
+                X[(active && (left || right)) ? x + (left ? -1 : 1) : dummy] = lr;
+                if (n == 1) {  // a single interior cell borders the ring on every side
+                    X[active ? x + XP : dummy] = tb;
+                    X[active ? x + 1 : dummy] = lr;
                 }
                 xprev = active ? v : xprev;
             }
