@@ -372,26 +372,31 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
                 const float wv = (c == 0) ? wh[q] : xprev;
                 const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, denom, inv);
                 X[active ? me0 + s : dummy] = v;
-                xprev = active ? v : xprev;
+                // xprev is consumed only where this lane was active one tick
+                // earlier (as W) or lane r + 1 is active now (as N, same
+                // column), so it need not be held across inactive ticks
+                xprev = v;
             }
         }
     } else {
-        // Units that meet the ring keep it as data: a relaxed edge cell also
-        // writes its sign-adjusted ring cell, in LDS for the next sweep (which
-        // reads it two steps later, like any other neighbour) and to the grid
-        // at write-back for the next unit.  Sweep 0 reads the stored ring.
+        // Units that meet the ring: a neighbour on the ring is, for sweep
+        // k > 0, the sign-adjusted previous-sweep value of the cell itself
+        // (read with the other operands; the cell is overwritten only after
+        // its own relaxation), and for sweep 0 the stored ring, which the
+        // unit loaded.  Write-back stores the ring of every edge cell, so the
+        // next unit finds the ring of the latest sweep there.
         const bool row_ok = wave_ok && i >= 1 && i <= n;
-        const bool top = (i == 1), bot = (i == n);
-        const bool n_from_lds = (r == 0) || top;
+        const bool top = (i == 1), bot = (i == n), kpos = (k > 0);
         const int c_left = 1 - jbase, c_right = n - jbase;  // skewed columns of j = 1 and j = n
         const int c_lo = max(0, c_left), c_hi = min(TW - 1, c_right);
-        // ring rows: one store per tick serves top or bottom (both only when n == 1)
-        const int tb_ofs = top ? -XP : XP;
+        const unsigned tb_sign = rg.neg_tb ? 0x80000000u : 0u, lr_sign = rg.neg_lr ? 0x80000000u : 0u;
+        const bool n_ring = top && kpos;                   // N is the virtual ring
+        const bool n_lds_lane = (r == 0) || top;           // else from LDS: tile halo, or stored ring (k = 0)
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
             if (!row_ok || s0 + L <= 0 || s0 >= STEPS) continue;
-            float s_lds[L], e_lds[L], bb[L], n_lds[L], w_lds[L];
+            float s_lds[L], e_lds[L], bb[L], n_lds[L], w_lds[L], self[L];
 #pragma unroll
             for (int q = 0; q < L; ++q) {
                 const int x = me0 + s0 + q;
@@ -400,6 +405,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
                 bb[q] = Bs[bme0 + s0 + q];
                 n_lds[q] = X[x - XP];
                 w_lds[q] = X[x - 1];
+                self[q] = X[x];
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -407,21 +413,16 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
                 const int s = s0 + q;
                 const int c = s - r;
                 const bool active = c >= c_lo && c <= c_hi;
-                const bool left = (c == c_left), right = (c == c_right);
                 const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                const float nv = n_from_lds ? n_lds[q] : from_up;
-                const float wv = (c == 0 || left) ? w_lds[q] : xprev;
-                const float v = relax_k<KIND>(bb[q], nv, s_lds[q], wv, e_lds[q], a, denom, inv);
-                const int x = me0 + s;
-                const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
-                X[active ? x : dummy] = v;
-                X[(active && (top || bot)) ? x + tb_ofs : dummy] = tb;
-                X[(active && (left || right)) ? x + (left ? -1 : 1) : dummy] = lr;
-                if (n == 1) {  // a single interior cell borders the ring on every side
-                    X[active ? x + XP : dummy] = tb;
-                    X[active ? x + 1 : dummy] = lr;
-                }
-                xprev = active ? v : xprev;
+                const float tbv = __uint_as_float(__float_as_uint(self[q]) ^ tb_sign);
+                const float lrv = __uint_as_float(__float_as_uint(self[q]) ^ lr_sign);
+                const float nv = n_ring ? tbv : (n_lds_lane ? n_lds[q] : from_up);
+                const float sv = (bot && kpos) ? tbv : s_lds[q];
+                const float wv = (c == c_left && kpos) ? lrv : ((c == 0 || c == c_left) ? w_lds[q] : xprev);
+                const float ev = (c == c_right && kpos) ? lrv : e_lds[q];
+                const float v = relax_k<KIND>(bb[q], nv, sv, wv, ev, a, denom, inv);
+                X[active ? me0 + s : dummy] = v;
+                xprev = v;
             }
         }
     }
