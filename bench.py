@@ -30,6 +30,19 @@ for p in (ROOT, os.path.join(ROOT, "fluid-2d_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_unit_pressure_4096_traffic.txt")
+
+
+def committed_traffic(order: int, n: int, K: int):
+    """HBM bytes of one pressure solve measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (calibrated against a plain copy) for the Gauss-Seidel
+    4096^2, K=80 configuration; None for any other configuration."""
+    if order != 0 or n != 4096 or K != 80 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_pressure_solve:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def bytes_per_cell_step(kd: int, kp: int, a_dens: bool, a_visc: bool) -> int:
@@ -148,14 +161,15 @@ def main() -> int:
         "bytes_per_cell_step": bpc,
         "launches_per_step": launches,
         "roofline": {
-            "kernel": "k_gs_tile<64,pressure>" if order == 0 else "k_jacobi_block",
+            "kernel": "k_gs_unit<64,8,4,pressure>" if order == 0 else "k_jacobi_block<4>",
             "bound": "hbm",
             "achieved": round(solve_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(solve_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "unit_of_work": f"one pressure solve: {K} sweeps x {cells} cells x 12 B, {ms_solve:.3f} ms",
+            "traffic": committed_traffic(order, n, K),
+            "unit_of_work": f"one pressure solve: {K} sweeps x {cells} cells x 12 B, {ms_solve:.3f} ms; "
+                            "achieved and traffic are per solve",
         },
     }
     if order == 0 and not args.no_jacobi:

@@ -285,9 +285,10 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     float* __restrict__ x = A.x;
     const Ring rg = A.ring;
 
-    // ---- load (all loads of a batch in flight before any LDS store)
+    // ---- load: iterate region and right-hand side in one batch, all loads in
+    // flight before any LDS store (one memory round trip per unit)
     {
-        float v[NX];
+        float v[NX], vb[NBX];
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int idx = min(tid + q * NT, XH * XW - 1);
@@ -295,23 +296,20 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
             v[q] = x[(size_t)gi * P + gj];
         }
 #pragma unroll
+        for (int q = 0; q < NBX; ++q) {
+            const int idx = min(tid + q * NT, BH * BW - 1);
+            const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
+            vb[q] = A.b[(size_t)gi * P + gj];
+        }
+#pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int idx = tid + q * NT;
             if (idx < XH * XW) X[(idx / XW) * XP + idx % XW] = v[q];
         }
-    }
-    {
-        float v[NBX];
-#pragma unroll
-        for (int q = 0; q < NBX; ++q) {
-            const int idx = min(tid + q * NT, BH * BW - 1);
-            const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
-            v[q] = A.b[(size_t)gi * P + gj];
-        }
 #pragma unroll
         for (int q = 0; q < NBX; ++q) {
             const int idx = tid + q * NT;
-            if (idx < BH * BW) Bs[(idx / BW) * BP + idx % BW] = v[q];
+            if (idx < BH * BW) Bs[(idx / BW) * BP + idx % BW] = vb[q];
         }
     }
 
