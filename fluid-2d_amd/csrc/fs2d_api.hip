@@ -41,6 +41,7 @@ struct fs2d_handle {
     int gs_block = 8;
     int gs_lag = 4;
     int jacobi_block = 4;
+    int jacobi_kernel = FS2D_JACOBI_WAVE;
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
     bool speed_valid = false;   // max_bits holds max |u|, |v| of the current velocity
     std::string err;
@@ -114,7 +115,7 @@ int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, f
         }
         FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
     } else {
-        FS2D_HIP(h, launch_jacobi_solve(h->stream, A, h->f[F_SCRATCH], h->jacobi_block));
+        FS2D_HIP(h, launch_jacobi_solve(h->stream, A, h->f[F_SCRATCH], h->jacobi_block, h->jacobi_kernel));
         if (jacobi_launches(K, h->jacobi_block) & 1) std::swap(x, h->f[F_SCRATCH]);
     }
     return FS2D_OK;
@@ -280,8 +281,8 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_lag must be 0, 1, 4 or 8");
     if (prm.gs_block > 1 && prm.gs_tile_cols == 128)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_block > 1 needs gs_tile_cols 64");
-    for (int r : prm.reserved)
-        if (r != 0) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: reserved fields must be 0");
+    if (prm.jacobi_kernel != 0 && prm.jacobi_kernel != FS2D_JACOBI_WAVE && prm.jacobi_kernel != FS2D_JACOBI_LDS)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: jacobi_kernel must be 0, 1 (wave) or 2 (LDS)");
 
     fs2d_handle* h = new fs2d_handle();
     h->R = rows;
@@ -292,6 +293,7 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     h->gs_block = prm.gs_block ? prm.gs_block : 8;
     h->gs_lag = prm.gs_lag ? prm.gs_lag : 4;
     h->jacobi_block = prm.jacobi_block ? prm.jacobi_block : 4;
+    h->jacobi_kernel = prm.jacobi_kernel ? prm.jacobi_kernel : FS2D_JACOBI_WAVE;
     auto bail = [&](hipError_t e, const char* what) {
         std::string msg = std::string("fs2d_hip_create: ") + what + ": " + hipGetErrorString(e);
         fs2d_hip_destroy(h);

@@ -98,9 +98,12 @@ hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a
 // All solves of the set in shared level launches (unit plans; a tile plan takes one).
 hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set);
 
-// Jacobi sweeps, `block` sweeps fused per launch; ping-pongs between x and
-// scratch and leaves the result in x.
-hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch, int block);
+// Jacobi sweeps, `block` sweeps fused per launch, ping-ponging x -> scratch ->
+// x -> ...; after an odd number of launches the result is in scratch.
+// kernel: JACOBI_WAVE (register strips, one wave per tile, DPP lane exchange)
+// or JACOBI_LDS (256-thread blocks, halo exchange in LDS); identical results.
+enum JacobiKernel : int { JACOBI_WAVE = 1, JACOBI_LDS = 2 };
+hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch, int block, int kernel);
 int jacobi_launches(int K, int block);
 
 }  // namespace fs2d

@@ -547,6 +547,168 @@ __global__ __launch_bounds__(JT) void k_jacobi_block(SweepArgs A, const float* _
     }
 }
 
+// ===================== Jacobi, register strips ===========================
+// One wave per tile and no LDS: lane l holds column gj0 + l of a 64-column
+// region, its TY + 2M rows in registers (iterate and right-hand side).  A
+// sweep walks the rows top to bottom: west and east come from lanes l -+ 1
+// by a full-wave DPP shift, north is the row's previous value (kept before it
+// is overwritten), south the next row's.  Each sweep leaves one more row and
+// lane at each side stale, so after M sweeps rows M .. M+TY-1 and lanes
+// M .. 63-M hold sweep M; those are written.  Region cells outside the grid
+// hold clamped values that only ever reach other outside cells or ring cells,
+// and ring cells inside the region are rebuilt from the new interior after
+// every sweep, so no test guards the interior update.
+constexpr int JW_WAVES = 4;  // independent tiles per block
+
+__device__ __forceinline__ float lane_west(float x) {  // lane l receives lane l - 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_east(float x) {  // lane l receives lane l + 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+
+template <int M, int TY, int KIND, bool EDGE>
+__device__ __forceinline__ void jacobi_strip(const SweepArgs& A, const float* __restrict__ src,
+                                             float* __restrict__ dst, bool zero_src, int gi0, int gj0, int lane) {
+    constexpr int H = TY + 2 * M;
+    const int R = A.R, P = A.P;
+    const int gj = gj0 + lane;
+    float X[H], B[H];
+    // ---- all loads in flight before the first use
+    if (EDGE) {
+        const int cj = min(max(gj, 0), R - 1), bj = min(max(gj, 1), R - 2);
+#pragma unroll
+        for (int q = 0; q < H; ++q)
+            X[q] = zero_src ? 0.f : src[(size_t)min(max(gi0 + q, 0), R - 1) * P + cj];
+#pragma unroll
+        for (int q = 1; q < H - 1; ++q) B[q] = A.b[(size_t)min(max(gi0 + q, 1), R - 2) * P + bj];
+    } else {
+        const size_t o = (size_t)gi0 * P + gj;
+#pragma unroll
+        for (int q = 0; q < H; ++q) X[q] = zero_src ? 0.f : src[o + (size_t)q * P];
+#pragma unroll
+        for (int q = 1; q < H - 1; ++q) B[q] = A.b[o + (size_t)q * P];
+    }
+    B[0] = B[H - 1] = 0.f;
+
+    const float a = A.a, denom = A.denom, inv = A.inv_denom;
+    const Ring rg = A.ring;
+    const bool left = gj == 0, right = gj == R - 1;
+    const bool cols_ring = EDGE && (gj0 <= 0 || gj0 + 63 >= R - 1);  // wave-uniform
+#pragma unroll
+    for (int s = 1; s <= M; ++s) {
+        float north = X[s - 1];
+#pragma unroll
+        for (int q = s; q < H - s; ++q) {
+            const float c = X[q];
+            float v = relax_k<KIND>(B[q], north, X[q + 1], lane_west(c), lane_east(c), a, denom, inv);
+            north = c;
+            if (EDGE && s < M) {
+                // ring of this sweep: the bottom ring row from the new row above,
+                // the top ring row (row q - 1) from the new first interior row,
+                // the side columns from the new neighbouring lane
+                const int gi = gi0 + q;
+                if (gi == R - 1) v = sgn(rg.neg_tb, X[q - 1]);
+                if (cols_ring) {
+                    const float e = sgn(rg.neg_lr, lane_east(v)), w = sgn(rg.neg_lr, lane_west(v));
+                    v = left ? e : (right ? w : v);
+                }
+                if (gi == 1) X[q - 1] = sgn(rg.neg_tb, v);
+            }
+            X[q] = v;
+        }
+    }
+
+    // ---- write rows M .. M+TY-1, lanes M .. 63-M: interior cells and the ring
+    // next to them
+    if (lane < M || lane >= 64 - M) return;
+    if (!EDGE) {
+        float* o = dst + (size_t)gi0 * P + gj;
+#pragma unroll
+        for (int q = M; q < M + TY; ++q) o[(size_t)q * P] = X[q];
+        return;
+    }
+    if (gj < 1 || gj > R - 2) return;
+    const int n = R - 2;
+#pragma unroll
+    for (int q = M; q < M + TY; ++q) {
+        const int gi = gi0 + q;
+        if (gi < 1 || gi > n) continue;
+        const float v = X[q];
+        const size_t g = (size_t)gi * P + gj;
+        dst[g] = v;
+        const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
+        const float cn = 0.5f * (tb + lr);
+        if (gi == 1) dst[g - P] = tb;
+        if (gi == n) dst[g + P] = tb;
+        if (gj == 1) dst[g - 1] = lr;
+        if (gj == n) dst[g + 1] = lr;
+        if (gi == 1 && gj == 1) dst[g - P - 1] = cn;
+        if (gi == 1 && gj == n) dst[g - P + 1] = cn;
+        if (gi == n && gj == 1) dst[g + P - 1] = cn;
+        if (gi == n && gj == n) dst[g + P + 1] = cn;
+    }
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; the first 8 * floor(nb / 8)
+// are renumbered so that each XCD walks a contiguous run of tiles (rows of
+// tiles in order), whose shared halo rows and columns then meet in its L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int full = nb - nb % 8;
+    return b < full ? (b % 8) * (full / 8) + b / 8 : b;
+}
+
+template <int M, int TY, int KIND>
+__global__ __launch_bounds__(64 * JW_WAVES) void k_jacobi_wave(SweepArgs A, const float* __restrict__ src,
+                                                              float* __restrict__ dst, int zero_src, int tiles_x,
+                                                              int tiles) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * JW_WAVES + threadIdx.x / 64;
+    if (t >= tiles) return;  // whole wave
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    constexpr int OX = 64 - 2 * M;
+    const int gi0 = 1 + TY * ty - M, gj0 = 1 + OX * tx - M;
+    const int R = A.R;
+    const bool plain = gi0 >= 1 && gi0 + TY + 2 * M - 1 <= R - 2 && gj0 >= 1 && gj0 + 63 <= R - 2;
+    const int lane = threadIdx.x & 63;
+    if (plain)
+        jacobi_strip<M, TY, KIND, false>(A, src, dst, zero_src != 0, gi0, gj0, lane);
+    else
+        jacobi_strip<M, TY, KIND, true>(A, src, dst, zero_src != 0, gi0, gj0, lane);
+}
+
+constexpr int JW_TY = 32;  // rows per tile
+
+template <int M>
+hipError_t launch_jw(hipStream_t s, const SweepArgs& a, const float* src, float* dst, int zero_src) {
+    constexpr int OX = 64 - 2 * M;
+    const int n = a.R - 2;
+    const int tiles_x = (n + OX - 1) / OX, tiles_y = (n + JW_TY - 1) / JW_TY;
+    const int tiles = tiles_x * tiles_y;
+    const dim3 grid((tiles + JW_WAVES - 1) / JW_WAVES);
+    if (a.kind == SWEEP_DIFFUSE)
+        hipLaunchKernelGGL((k_jacobi_wave<M, JW_TY, SWEEP_DIFFUSE>), grid, dim3(64 * JW_WAVES), 0, s, a, src, dst,
+                           zero_src, tiles_x, tiles);
+    else
+        hipLaunchKernelGGL((k_jacobi_wave<M, JW_TY, SWEEP_PRESSURE>), grid, dim3(64 * JW_WAVES), 0, s, a, src, dst,
+                           zero_src, tiles_x, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_wave(hipStream_t s, const SweepArgs& a, const float* src, float* dst, int zero_src,
+                              int m) {
+    switch (m) {
+        case 1: return launch_jw<1>(s, a, src, dst, zero_src);
+        case 2: return launch_jw<2>(s, a, src, dst, zero_src);
+        case 3: return launch_jw<3>(s, a, src, dst, zero_src);
+        case 4: return launch_jw<4>(s, a, src, dst, zero_src);
+        case 5: return launch_jw<5>(s, a, src, dst, zero_src);
+        case 6: return launch_jw<6>(s, a, src, dst, zero_src);
+        case 7: return launch_jw<7>(s, a, src, dst, zero_src);
+        case 8: return launch_jw<8>(s, a, src, dst, zero_src);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <int M>
 hipError_t launch_jb(hipStream_t s, const SweepArgs& a, const float* src, float* dst, int zero_src) {
     const dim3 grid((a.R - 2 + JX - 1) / JX, (a.R - 2 + JY - 1) / JY);
@@ -734,16 +896,17 @@ hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a
 // ---- Jacobi ------------------------------------------------------------------
 int jacobi_launches(int K, int block) { return (K + block - 1) / block; }
 
-hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch, int block) {
+hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch, int block, int kernel) {
     // Sweep order: x -> scratch -> x -> ...; an odd launch count ends in
-    // scratch, and the caller swaps the two buffers (see the API layer).
+    // scratch, and the caller swaps the two buffers.
     const float* src = a.x;
     float* dst = scratch;
     int done = 0;
     bool zero = a.zero_start != 0;
     while (done < a.K) {
         const int m = std::min(block, a.K - done);
-        hipError_t e = launch_jacobi_block(s, a, src, dst, zero ? 1 : 0, m);
+        hipError_t e = kernel == JACOBI_LDS ? launch_jacobi_block(s, a, src, dst, zero ? 1 : 0, m)
+                                            : launch_jacobi_wave(s, a, src, dst, zero ? 1 : 0, m);
         if (e != hipSuccess) return e;
         zero = false;
         done += m;

@@ -18,6 +18,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libfs2d_hip.so")
 
 GS_LEX, JACOBI = 0, 1
+JACOBI_WAVE, JACOBI_LDS = 1, 2  # fs2d_params.jacobi_kernel
 DENSITY, VELOCITY_U, VELOCITY_V = 0, 1, 2
 
 _f, _i, _p = ctypes.c_float, ctypes.c_int, ctypes.c_void_p
@@ -30,7 +31,7 @@ class Fs2dError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("diffuse_iterations", _i), ("pressure_iterations", _i), ("sweep_order", _i),
                 ("jacobi_block", _i), ("gs_tile_cols", _i), ("gs_block", _i), ("gs_lag", _i),
-                ("reserved", _i * 1)]
+                ("jacobi_kernel", _i)]
 
 
 _lib = None
@@ -94,7 +95,7 @@ class Solver:
 
     def __init__(self, rows: int, cols: int | None = None, diffuse_iterations: int = 20,
                  pressure_iterations: int = 20, order: int = GS_LEX, jacobi_block: int = 0,
-                 gs_tile_cols: int = 0, gs_block: int = 0, gs_lag: int = 0):
+                 gs_tile_cols: int = 0, gs_block: int = 0, gs_lag: int = 0, jacobi_kernel: int = 0):
         L = library()
         cols = rows if cols is None else cols
         prm = Params()
@@ -106,6 +107,7 @@ class Solver:
         prm.gs_tile_cols = gs_tile_cols
         prm.gs_block = gs_block
         prm.gs_lag = gs_lag
+        prm.jacobi_kernel = jacobi_kernel
         h = _p()
         rc = L.fs2d_hip_create(rows, cols, ctypes.byref(prm), ctypes.byref(h))
         if rc != 0:

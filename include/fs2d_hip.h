@@ -46,6 +46,12 @@ enum fs2d_sweep_order {
     FS2D_ORDER_JACOBI = 1
 };
 
+/* Kernel that runs the Jacobi sweeps (identical results). */
+enum fs2d_jacobi_kernel {
+    FS2D_JACOBI_WAVE = 1, /* one wave per tile, rows in registers, lanes exchange by DPP (default) */
+    FS2D_JACOBI_LDS = 2   /* 256-thread blocks, halo exchange through LDS                         */
+};
+
 /* Boundary rule of a field (src/fluid_solver_cpu.cpp:33-83). */
 enum fs2d_field {
     FS2D_FIELD_DENSITY = 0,    /* continuous ring                    */
@@ -61,7 +67,7 @@ typedef struct fs2d_params {
     int gs_tile_cols;        /* Gauss-Seidel tile width in cells; 0 = default                            */
     int gs_block;            /* Gauss-Seidel sweeps per tile (temporal block): 1, 2, 4 or 8; 0 = default */
     int gs_lag;              /* steps each sweep trails the previous one in a tile: 1, 4 or 8; 0 = default */
-    int reserved[1];         /* must be 0                                                                */
+    int jacobi_kernel;       /* enum fs2d_jacobi_kernel; 0 = default (register strips)                   */
 } fs2d_params;
 
 typedef struct fs2d_handle fs2d_handle;

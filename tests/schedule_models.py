@@ -596,3 +596,78 @@ def gs_unit_ring_model(x, b, K, kind, a, denom, neg_tb, neg_lr, M, L, TW=64):
             for (i, j), val in out.items():
                 x[i, j] = val
     return x
+
+
+def jacobi_wave_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, TY=64):
+    """Jacobi solve by k_jacobi_wave launches: one 64-lane wave per tile, lane =
+    column of a 64-column region (output 64 - 2M columns), rows held in
+    registers; per sweep, the region shrinks by one cell on each side, W/E
+    come from the neighbouring lanes and N/S from the neighbouring rows, and
+    ring cells inside the region are rewritten from the new interior.
+    Returns the result array (the kernel's x or scratch)."""
+    R = x.shape[0]
+    a, denom = F(a), F(denom)
+    src = x.copy()
+    zero = bool(zero_start)
+    done = 0
+    while done < K:
+        m = min(M, K - done)  # a remainder launch runs with its own margin
+        OX = 64 - 2 * m
+        dst = np.full((R, R), np.nan, np.float32)
+        H = TY + 2 * m
+        for ty in range(-(-(R - 2) // TY)):
+            for tx in range(-(-(R - 2) // OX)):
+                gi0, gj0 = 1 + TY * ty - m, 1 + OX * tx - m  # region origin
+                gis = [gi0 + q for q in range(H)]
+                gjs = [gj0 + l for l in range(64)]
+                cl = lambda v, lo, hi: min(max(v, lo), hi)
+                X = np.array([[F(0) if zero else src[cl(gi, 0, R - 1), cl(gj, 0, R - 1)] for gj in gjs] for gi in gis],
+                             np.float32)
+                B = np.array([[b[cl(gi, 1, R - 2), cl(gj, 1, R - 2)] for gj in gjs] for gi in gis], np.float32)
+                valid = np.array([[0 <= gi < R and 0 <= gj < R for gj in gjs] for gi in gis])
+                for s in range(1, m + 1):
+                    new = X.copy()
+                    for q in range(s, H - s):
+                        for l in range(s, 64 - s):
+                            gi, gj = gis[q], gjs[l]
+                            if not (1 <= gi <= R - 2 and 1 <= gj <= R - 2):
+                                continue
+                            nb = (X[q - 1, l], X[q + 1, l], X[q, l - 1], X[q, l + 1])
+                            for (qq, ll) in ((q - 1, l), (q + 1, l), (q, l - 1), (q, l + 1)):
+                                assert valid[qq, ll]
+                            new[q, l] = _relax(kind, B[q, l], nb[0], nb[1], nb[2], nb[3], a, denom)
+                    # ring cells of the region, from the new interior (rows, then columns)
+                    for q in range(s, H - s):
+                        for l in range(s, 64 - s):
+                            gi, gj = gis[q], gjs[l]
+                            if gi == 0 and 1 <= gj <= R - 2: new[q, l] = _sgn(neg_tb, new[q + 1, l])
+                            if gi == R - 1 and 1 <= gj <= R - 2: new[q, l] = _sgn(neg_tb, new[q - 1, l])
+                    for q in range(s, H - s):
+                        for l in range(s, 64 - s):
+                            gi, gj = gis[q], gjs[l]
+                            if gj == 0 and 1 <= gi <= R - 2: new[q, l] = _sgn(neg_lr, new[q, l + 1])
+                            if gj == R - 1 and 1 <= gi <= R - 2: new[q, l] = _sgn(neg_lr, new[q, l - 1])
+                    X = new
+                # output: interior cells of the tile and the ring next to them
+                for q in range(m, m + TY):
+                    for l in range(m, m + OX):
+                        gi, gj = gis[q], gjs[l]
+                        if not (1 <= gi <= R - 2 and 1 <= gj <= R - 2):
+                            continue
+                        v = X[q, l]
+                        dst[gi, gj] = v
+                        tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                        cn = F(F(0.5) * F(tb + lr))
+                        if gi == 1: dst[0, gj] = tb
+                        if gi == R - 2: dst[R - 1, gj] = tb
+                        if gj == 1: dst[gi, 0] = lr
+                        if gj == R - 2: dst[gi, R - 1] = lr
+                        if gi == 1 and gj == 1: dst[0, 0] = cn
+                        if gi == 1 and gj == R - 2: dst[0, R - 1] = cn
+                        if gi == R - 2 and gj == 1: dst[R - 1, 0] = cn
+                        if gi == R - 2 and gj == R - 2: dst[R - 1, R - 1] = cn
+        assert not np.isnan(dst).any(), "unwritten cells after a Jacobi launch"
+        src = dst
+        zero = False
+        done += m
+    return src

@@ -36,10 +36,10 @@ def assert_same_nan(got, want, what):
     assert bits_equal(g, w), f"{what}: {describe(g, w)}"
 
 
-def make(R, state, order=GS, kd=20, kp=20, block=0, gsb=0, lag=0):
+def make(R, state, order=GS, kd=20, kp=20, block=0, gsb=0, lag=0, jk=0):
     rho, u, v, rs, us, vs = state
     s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order, jacobi_block=block,
-                        gs_block=gsb, gs_lag=lag)
+                        gs_block=gsb, gs_lag=lag, jacobi_kernel=jk)
     s.upload(rho, u, v)
     s.set_sources(rs, us, vs)
     return s
@@ -75,18 +75,22 @@ def test_add_sources(R, oracle_lib):
         assert_same(got[f], want, f"add_sources R={R} field={f}")
 
 
-# (order, jacobi_block, gs_block, gs_lag)
-GS_VARIANTS = [(GS, 0, 1, 0), (GS, 0, 2, 1), (GS, 0, 2, 8), (GS, 0, 4, 4), (GS, 0, 8, 1), (GS, 0, 8, 8)]
-SWEEP_VARIANTS = GS_VARIANTS + [(JAC, 1, 0, 0), (JAC, 3, 0, 0), (JAC, 4, 0, 0), (JAC, 8, 0, 0)]
+WAVE, LDS = fs2d_amd.JACOBI_WAVE, fs2d_amd.JACOBI_LDS
+# (order, jacobi_block, gs_block, gs_lag, jacobi_kernel)
+GS_VARIANTS = [(GS, 0, 1, 0, 0), (GS, 0, 2, 1, 0), (GS, 0, 2, 8, 0), (GS, 0, 4, 4, 0), (GS, 0, 8, 1, 0),
+               (GS, 0, 8, 8, 0)]
+JAC_VARIANTS = [(JAC, 1, 0, 0, WAVE), (JAC, 3, 0, 0, WAVE), (JAC, 4, 0, 0, WAVE), (JAC, 8, 0, 0, WAVE),
+                (JAC, 1, 0, 0, LDS), (JAC, 4, 0, 0, LDS), (JAC, 8, 0, 0, LDS)]
+SWEEP_VARIANTS = GS_VARIANTS + JAC_VARIANTS
 
 
-@pytest.mark.parametrize("order,block,gsb,lag", SWEEP_VARIANTS)
+@pytest.mark.parametrize("order,block,gsb,lag,jk", SWEEP_VARIANTS)
 @pytest.mark.parametrize("R", [4, 10, 67, 130, 258])
 @pytest.mark.parametrize("rate", [0.0, 1e-3, 5e-2])
-def test_diffuse(R, rate, order, block, gsb, lag, oracle_lib):
+def test_diffuse(R, rate, order, block, gsb, lag, jk, oracle_lib):
     st = W.random_state(R, seed=11 * R)
     for K in (1, 2, 7, 20):
-        with make(R, st, order=order, block=block, gsb=gsb, lag=lag) as s:
+        with make(R, st, order=order, block=block, gsb=gsb, lag=lag, jk=jk) as s:
             for f in range(3):
                 s.diffuse(f, rate, 0.1, K)
             got = s.download()
@@ -94,7 +98,7 @@ def test_diffuse(R, rate, order, block, gsb, lag, oracle_lib):
             want = st[f].copy()
             oracle_lib.diffuse(want, f, rate, 0.1, K, oracle_order(order))
             assert_same(got[f], want, f"diffuse R={R} field={f} rate={rate} K={K} order={order} "
-                                      f"block={block} gs_block={gsb} lag={lag}")
+                                      f"block={block} gs_block={gsb} lag={lag} jk={jk}")
 
 
 @pytest.mark.parametrize("R", SIZES)
@@ -151,18 +155,20 @@ def test_non_finite_velocity(oracle_lib):
         assert_same_nan(got[f], want[f], f"non-finite field={f}")
 
 
-@pytest.mark.parametrize("order,block,gsb,lag", GS_VARIANTS + [(JAC, 2, 0, 0), (JAC, 4, 0, 0), (JAC, 5, 0, 0)])
+@pytest.mark.parametrize("order,block,gsb,lag,jk", GS_VARIANTS + [(JAC, 2, 0, 0, WAVE), (JAC, 4, 0, 0, WAVE),
+                                                                  (JAC, 5, 0, 0, WAVE), (JAC, 8, 0, 0, WAVE),
+                                                                  (JAC, 4, 0, 0, LDS)])
 @pytest.mark.parametrize("R", [3, 4, 10, 67, 130, 258])
-def test_project(R, order, block, gsb, lag, oracle_lib):
+def test_project(R, order, block, gsb, lag, jk, oracle_lib):
     st = W.random_state(R, seed=7 * R, vel_scale=1.0)
     for K in (0, 1, 3, 20):
-        with make(R, st, order=order, block=block, gsb=gsb, lag=lag) as s:
+        with make(R, st, order=order, block=block, gsb=gsb, lag=lag, jk=jk) as s:
             s.project(K)
             _, gu, gv = s.download()
             gp = s.pressure()
         wu, wv = st[1].copy(), st[2].copy()
         wp = oracle_lib.project(wu, wv, K, oracle_order(order), want_p=True)
-        assert_same(gp, wp, f"pressure R={R} K={K} order={order} gs_block={gsb} lag={lag}")
+        assert_same(gp, wp, f"pressure R={R} K={K} order={order} gs_block={gsb} lag={lag} block={block} jk={jk}")
         assert_same(gu, wu, f"project u R={R} K={K} order={order}")
         assert_same(gv, wv, f"project v R={R} K={K} order={order}")
 
@@ -242,15 +248,16 @@ def test_config1_against_reference():
 
 
 # ------------------------------------------------ whole steps vs the oracle
-@pytest.mark.parametrize("order,block,gsb,lag", GS_VARIANTS + [(JAC, 4, 0, 0)])
+@pytest.mark.parametrize("order,block,gsb,lag,jk", GS_VARIANTS + [(JAC, 4, 0, 0, WAVE), (JAC, 8, 0, 0, WAVE),
+                                                                  (JAC, 4, 0, 0, LDS)])
 @pytest.mark.parametrize("R", [67, 130, 258])
-def test_steps_vs_oracle(R, order, block, gsb, lag, oracle_lib):
+def test_steps_vs_oracle(R, order, block, gsb, lag, jk, oracle_lib):
     st = W.vortex_state(R, seed=R, n_vortices=16)
     rs, us, vs = W.disc_sources(R, radius=R / 16)
     rho, u, v = (x.copy() for x in st)
     want = [x.copy() for x in st]
     with fs2d_amd.Solver(R, diffuse_iterations=12, pressure_iterations=30, order=order, jacobi_block=block,
-                         gs_block=gsb, gs_lag=lag) as s:
+                         gs_block=gsb, gs_lag=lag, jacobi_kernel=jk) as s:
         s.upload(rho, u, v)
         s.set_sources(rs, us, vs)
         done = 0
@@ -262,7 +269,7 @@ def test_steps_vs_oracle(R, order, block, gsb, lag, oracle_lib):
             done = target
             got = s.download()
             for f, name in enumerate(("density", "u", "v")):
-                assert_same(got[f], want[f], f"{name} R={R} order={order} after {target} steps")
+                assert_same(got[f], want[f], f"{name} R={R} order={order} block={block} jk={jk} after {target} steps")
 
 
 def test_config2_one_step(oracle_lib):

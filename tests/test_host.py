@@ -16,7 +16,7 @@ import oracle
 from conftest import ROOT, bits_equal
 from fs2d_amd import workloads as W
 from schedule_models import (gs_tile_model, gs_unit_lag_model, gs_unit_model, gs_unit_ring_model,
-                             jacobi_block_model)
+                             jacobi_block_model, jacobi_wave_model)
 
 
 def header_symbols():
@@ -51,6 +51,7 @@ def test_default_params_are_the_reference_literals():
     (10, 10, {"gs_tile_cols": 48}, "gs_tile_cols"),
     (10, 10, {"gs_block": 3}, "gs_block"),
     (10, 10, {"gs_lag": 2}, "gs_lag"),
+    (10, 10, {"jacobi_kernel": 3}, "jacobi_kernel"),
 ])
 def test_create_validates_arguments_before_touching_the_device(rows, cols, kw, msg):
     with pytest.raises(fs2d_amd.Fs2dError, match=msg):
@@ -161,6 +162,26 @@ def test_jacobi_block_schedule_reproduces_jacobi(R, M, oracle_lib):
     aa, bb = u.copy(), v.copy()
     p_ref = oracle_lib.project(aa, bb, 4, oracle.JACOBI, want_p=True)
     p = jacobi_block_model(np.full((R, R), 7.0, np.float32), d, 4, 1, 0, 0, False, False, 1, M)
+    assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M,TY", [(3, 2, 8), (4, 4, 8), (10, 3, 8), (34, 4, 16), (67, 2, 32), (70, 8, 16)])
+def test_jacobi_wave_schedule_reproduces_jacobi(R, M, TY, oracle_lib):
+    """k_jacobi_wave's tiling: register strips of TY + 2M rows x 64 lanes,
+    region shrinking by one row and lane per sweep, rings rebuilt in the
+    region after every sweep; remainder launches use their own margin."""
+    rho, u, v, *_ = W.random_state(R, seed=R)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        for K in (1, 5, 9):
+            y = jacobi_wave_model(rho.copy(), rho.copy(), K, 0, a, den, mode == 2, mode == 1, 0, M, TY)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.JACOBI)
+            assert bits_equal(y, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.JACOBI, want_p=True)
+    p = jacobi_wave_model(np.full((R, R), 7.0, np.float32), d, 9, 1, 0, 0, False, False, 1, M, TY)
     assert bits_equal(p, p_ref)
 
 
