@@ -40,8 +40,8 @@ struct fs2d_handle {
     int gs_tw = 64;
     int gs_block = 8;
     int gs_lag = 4;
-    int jacobi_block = 4;
     int jacobi_kernel = FS2D_JACOBI_WAVE;
+    std::map<std::pair<int, int>, int> jacobi_depth;  // (kind, K) -> sweeps per launch, when tuned
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
     bool speed_valid = false;   // max_bits holds max |u|, |v| of the current velocity
     std::string err;
@@ -97,13 +97,70 @@ hipError_t gs_plan_for(fs2d_handle* h, int K, const GsPlan** out) {
     return hipSuccess;
 }
 
+constexpr int kJacobiDefaultDepth = 4;
+
+// Sweeps fused per Jacobi launch: the fixed jacobi_block if one was given,
+// otherwise the depth in {1, 2, 4, 8} with the least measured time per solve,
+// chosen on the first solve of each (kind, K): one launch of each depth is
+// timed (best of two, after a warm-up) reading the live iterate and writing
+// only the scratch field.  Results are identical for every depth.  A stream
+// under capture cannot be timed; the default depth is used then (not kept).
+int jacobi_depth_for(fs2d_handle* h, const SweepArgs& A, int* out) {
+    if (h->prm.jacobi_block) {
+        *out = h->prm.jacobi_block;
+        return FS2D_OK;
+    }
+    const auto key = std::make_pair(A.kind, A.K);
+    auto it = h->jacobi_depth.find(key);
+    if (it != h->jacobi_depth.end()) {
+        *out = it->second;
+        return FS2D_OK;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FS2D_HIP(h, hipStreamIsCapturing(h->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+        *out = kJacobiDefaultDepth;
+        return FS2D_OK;
+    }
+    float best = 0.f;
+    int best_m = kJacobiDefaultDepth;
+    for (int m : {1, 2, 4, 8}) {
+        if (m > 1 && m > A.K) break;
+        float t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
+            FS2D_HIP(h, launch_jacobi_pass(h->stream, A, A.x, h->f[F_SCRATCH], A.zero_start, m, h->jacobi_kernel));
+            FS2D_HIP(h, hipEventRecord(h->ev1, h->stream));
+            FS2D_HIP(h, hipEventSynchronize(h->ev1));
+            float ms = 0.f;
+            FS2D_HIP(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+            if (rep == 1 || (rep > 1 && ms < t)) t = ms;
+        }
+        const float cost = t * static_cast<float>(jacobi_launches(A.K, m));
+        if (m == 1 || cost < best) {
+            best = cost;
+            best_m = m;
+        }
+    }
+    h->jacobi_depth.emplace(key, best_m);
+    *out = best_m;
+    return FS2D_OK;
+}
+
+// Sweeps per Jacobi launch without tuning (for reporting): the fixed or tuned
+// depth, else the default.
+int jacobi_depth_known(const fs2d_handle* h, int kind, int K) {
+    if (h->prm.jacobi_block) return h->prm.jacobi_block;
+    auto it = h->jacobi_depth.find(std::make_pair(kind, K));
+    return it != h->jacobi_depth.end() ? it->second : kJacobiDefaultDepth;
+}
+
 // K relaxation sweeps of x against b; Jacobi may leave the result in the
 // scratch field, in which case the two pointers are swapped.
 int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, float denom, int mode, int K,
                  int zero_start) {
     if (K <= 0) return FS2D_OK;
-    SweepArgs A{x, b, h->R, h->P, kind, a, denom, kind == SWEEP_DIFFUSE ? 1.0f / denom : 0.f, ring_of(mode), K,
-                zero_start};
+    SweepArgs A = sweep_args(x, b, h->R, h->P, kind, a, denom, mode, K, zero_start);
     if (h->prm.sweep_order == FS2D_ORDER_GS_LEX) {
         const GsPlan* plan = nullptr;
         FS2D_HIP(h, gs_plan_for(h, K, &plan));
@@ -115,8 +172,10 @@ int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, f
         }
         FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
     } else {
-        FS2D_HIP(h, launch_jacobi_solve(h->stream, A, h->f[F_SCRATCH], h->jacobi_block, h->jacobi_kernel));
-        if (jacobi_launches(K, h->jacobi_block) & 1) std::swap(x, h->f[F_SCRATCH]);
+        int depth = kJacobiDefaultDepth, rc;
+        if ((rc = jacobi_depth_for(h, A, &depth))) return rc;
+        FS2D_HIP(h, launch_jacobi_solve(h->stream, A, h->f[F_SCRATCH], depth, h->jacobi_kernel));
+        if (jacobi_launches(K, depth) & 1) std::swap(x, h->f[F_SCRATCH]);
     }
     return FS2D_OK;
 }
@@ -187,9 +246,9 @@ int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
         const float dd = 1.f + 4.f * ad, dv = 1.f + 4.f * av;
         SweepSet set{};
         set.count = 3;
-        set.a[0] = SweepArgs{h->rho(), h->f[F_X0], R, P, SWEEP_DIFFUSE, ad, dd, 1.0f / dd, ring_of(BND_CONT), Kd, 0};
-        set.a[1] = SweepArgs{h->f[F_UN], h->f[F_X0U], R, P, SWEEP_DIFFUSE, av, dv, 1.0f / dv, ring_of(BND_OPP_H), Kd, 0};
-        set.a[2] = SweepArgs{h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, dv, 1.0f / dv, ring_of(BND_OPP_V), Kd, 0};
+        set.a[0] = sweep_args(h->rho(), h->f[F_X0], R, P, SWEEP_DIFFUSE, ad, dd, BND_CONT, Kd, 0);
+        set.a[1] = sweep_args(h->f[F_UN], h->f[F_X0U], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_H, Kd, 0);
+        set.a[2] = sweep_args(h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_V, Kd, 0);
         const GsPlan* plan = nullptr;
         FS2D_HIP(h, gs_plan_for(h, Kd, &plan));
         FS2D_HIP(h, launch_gs_solve_set(h->stream, *plan, set));
@@ -292,7 +351,6 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     h->gs_tw = prm.gs_tile_cols ? prm.gs_tile_cols : 64;
     h->gs_block = prm.gs_block ? prm.gs_block : 8;
     h->gs_lag = prm.gs_lag ? prm.gs_lag : 4;
-    h->jacobi_block = prm.jacobi_block ? prm.jacobi_block : 4;
     h->jacobi_kernel = prm.jacobi_kernel ? prm.jacobi_kernel : FS2D_JACOBI_WAVE;
     auto bail = [&](hipError_t e, const char* what) {
         std::string msg = std::string("fs2d_hip_create: ") + what + ": " + hipGetErrorString(e);
@@ -498,14 +556,14 @@ int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
 int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
     if (!h || !launches) return fail(h, FS2D_ERR_ARGUMENT, "launches_per_step: bad argument");
     const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
-    auto solve_launches = [&](int K) -> int {
+    auto solve_launches = [&](int K, int kind) -> int {
         if (K <= 0) return 0;
-        if (h->prm.sweep_order == FS2D_ORDER_JACOBI) return jacobi_launches(K, h->jacobi_block);
+        if (h->prm.sweep_order == FS2D_ORDER_JACOBI) return jacobi_launches(K, jacobi_depth_known(h, kind, K));
         const GsPlan* plan = nullptr;
         if (gs_plan_for(h, K, &plan) != hipSuccess) return -1;
         return plan->levels;
     };
-    const int d = solve_launches(Kd), p = solve_launches(Kp);
+    const int d = solve_launches(Kd, SWEEP_DIFFUSE), p = solve_launches(Kp, SWEEP_PRESSURE);
     if (d < 0 || p < 0) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
     // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather
     const bool shared = h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block > 1;

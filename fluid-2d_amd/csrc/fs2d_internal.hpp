@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 namespace fs2d {
@@ -44,7 +46,27 @@ struct SweepArgs {
     Ring ring;
     int K;           // total sweeps of this solve
     int zero_start;  // iterate starts at 0 everywhere, ring included (pressure, :186)
+    float div_lo;    // smallest nonzero |dividend| the FMA-corrected division takes
 };
+
+// Lower end of the dividends the FMA-corrected division handles (zero aside):
+// |x| >= 2^-100 keeps its remainder exactly representable and |x| >= d 2^-124
+// keeps the quotient normal; above 2^100 the denominator's reciprocal nears
+// the subnormal range, and every dividend takes the IEEE division.
+inline float division_floor(float d) {
+    if (!(d <= 0x1p100f)) return __builtin_inff();
+    return std::max(0x1p-100f, d * 0x1p-124f);
+}
+
+inline SweepArgs sweep_args(float* x, const float* b, int R, int P, int kind, float a, float denom, int mode, int K,
+                            int zero_start) {
+    SweepArgs A{x, b, R, P, kind, a, denom, 0.f, ring_of(mode), K, zero_start, __builtin_inff()};
+    if (kind == SWEEP_DIFFUSE) {
+        A.inv_denom = 1.0f / denom;
+        A.div_lo = division_floor(denom);
+    }
+    return A;
+}
 
 // Up to three independent solves with the same sweep count, relaxed by the
 // same level launches (e.g. the density, u and v diffuses of one step).
@@ -104,6 +126,9 @@ hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet
 // or JACOBI_LDS (256-thread blocks, halo exchange in LDS); identical results.
 enum JacobiKernel : int { JACOBI_WAVE = 1, JACOBI_LDS = 2 };
 hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch, int block, int kernel);
+// One launch of m fused Jacobi sweeps from src into dst (m in 1..8).
+hipError_t launch_jacobi_pass(hipStream_t s, const SweepArgs& a, const float* src, float* dst, int zero_src, int m,
+                              int kernel);
 int jacobi_launches(int K, int block);
 
 }  // namespace fs2d

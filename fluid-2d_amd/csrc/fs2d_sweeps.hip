@@ -36,26 +36,34 @@ namespace {
 
 __device__ __forceinline__ float sgn(bool neg, float x) { return neg ? -x : x; }
 
+// The divisor of a diffuse solve: d = 1 + 4a, y = RN(1/d), and lo, the
+// smallest nonzero |dividend| the FMA-corrected division takes (the host's
+// division_floor: max(2^-100, d 2^-124), or +inf when d > 2^100).
+struct Divisor {
+    float d, y, lo;
+};
+
+__device__ __forceinline__ Divisor divisor_of(const SweepArgs& A) { return Divisor{A.denom, A.inv_denom, A.div_lo}; }
+
 // x / d correctly rounded, given y = RN(1/d): q0 = RN(x y) is within ~1.5 ulp,
 // one FMA correction makes it faithful, and Markstein's step from a faithful
 // quotient (remainder exact by FMA) rounds correctly (P. Markstein, IBM J. R&D
-// 34(1), 1990; Muller et al., Handbook of Floating-Point Arithmetic).  Dividends
-// outside [2^-100, 2^100] (zero, tiny, huge, inf, NaN) take the IEEE division,
-// so no intermediate underflows or overflows.  Pinned against exact rational
-// arithmetic in tests/test_division.py.
-__device__ __forceinline__ float div_rn(float x, float d, float y) {
+// 34(1), 1990; Muller et al., Handbook of Floating-Point Arithmetic), for
+// lo <= |x| <= 2^100.  Other dividends (zero, tiny, huge, inf, NaN) take the
+// IEEE division.  Pinned against exact rational arithmetic in
+// tests/test_division.py.
+__device__ __forceinline__ float div_rn(float x, Divisor v) {
     const float ax = fabsf(x);
-    if (!(ax >= 0x1p-100f && ax <= 0x1p100f)) return x / d;
-    const float q0 = x * y;
-    const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, d, x), y, q0);
-    return __builtin_fmaf(__builtin_fmaf(-q1, d, x), y, q1);
+    if (!(ax >= v.lo && ax <= 0x1p100f)) return x / v.d;
+    const float q0 = x * v.y;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, v.d, x), v.y, q0);
+    return __builtin_fmaf(__builtin_fmaf(-q1, v.d, x), v.y, q1);
 }
 
 // The update of one cell; operand order as in src/fluid_solver_cpu.cpp:107-108
 // (diffuse: N + S + W + E) and :199-200 (pressure: d + E + W + S + N).
-__device__ __forceinline__ float relax(int kind, float b, float n, float s, float w, float e, float a, float denom,
-                                       float inv) {
-    if (kind == SWEEP_DIFFUSE) return div_rn(b + a * (((n + s) + w) + e), denom, inv);
+__device__ __forceinline__ float relax(int kind, float b, float n, float s, float w, float e, float a, Divisor dv) {
+    if (kind == SWEEP_DIFFUSE) return div_rn(b + a * (((n + s) + w) + e), dv);
     return ((((b + e) + w) + s) + n) / 4.0f;
 }
 
@@ -84,9 +92,8 @@ __device__ __forceinline__ const float* ring_source(const float* x, int gi, int 
 }
 
 template <int KIND>
-__device__ __forceinline__ float relax_k(float b, float n, float s, float w, float e, float a, float denom,
-                                         float inv) {
-    if (KIND == SWEEP_DIFFUSE) return div_rn(b + a * (((n + s) + w) + e), denom, inv);
+__device__ __forceinline__ float relax_k(float b, float n, float s, float w, float e, float a, Divisor dv) {
+    if (KIND == SWEEP_DIFFUSE) return div_rn(b + a * (((n + s) + w) + e), dv);
     return ((((b + e) + w) + s) + n) / 4.0f;
 }
 
@@ -187,7 +194,8 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
     // lane r + 1 overwrites cell (r + 1, c + 1) only at step s + 2), so only
     // xprev and the DPP shift carry the dependency from step to step.
     // Inactive lanes compute on clamped operands and discard the result.
-    const float a = A.a, denom = A.denom, inv = A.inv_denom;
+    const float a = A.a;
+    const Divisor dv = divisor_of(A);
     const int steps = rows + cols - 1;
     const bool lane_ok = lane < rows;
     const float w_halo = T[(lane + 1) * TP];
@@ -209,7 +217,7 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
         const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
         const float n = (lane == 0) ? n_halo : from_up;
         const float w = (c == 0) ? w_halo : xprev;
-        const float v = relax_k<KIND>(bb, n, so, w, e, a, denom, inv);
+        const float v = relax_k<KIND>(bb, n, so, w, e, a, dv);
         if (active) trow[cw + 1] = v;
         xprev = active ? v : xprev;
     }
@@ -320,7 +328,8 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     const bool wave_ok = (k <= k_hi);
     const int jbase = TW * J + 1 - k;
     const int ofs = M - w;  // X column of skewed column 0 is 1 + (M - 1 - w)
-    const float a = A.a, denom = A.denom, inv = A.inv_denom;
+    const float a = A.a;
+    const Divisor dv = divisor_of(A);
     float xprev = 0.f;
     // A unit whose cells all lie in rows and columns 2..n-1 never meets the
     // ring: its sweep needs no clamping (the LDS arrays carry guard space for
@@ -368,7 +377,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
                 const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
                 const float nv = (r == 0) ? nh[q] : from_up;
                 const float wv = (c == 0) ? wh[q] : xprev;
-                const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, denom, inv);
+                const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, dv);
                 X[active ? me0 + s : dummy] = v;
                 // xprev is consumed only where this lane was active one tick
                 // earlier (as W) or lane r + 1 is active now (as N, same
@@ -418,7 +427,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
                 const float sv = (bot && kpos) ? tbv : s_lds[q];
                 const float wv = (c == c_left && kpos) ? lrv : ((c == 0 || c == c_left) ? w_lds[q] : xprev);
                 const float ev = (c == c_right && kpos) ? lrv : e_lds[q];
-                const float v = relax_k<KIND>(bb[q], nv, sv, wv, ev, a, denom, inv);
+                const float v = relax_k<KIND>(bb[q], nv, sv, wv, ev, a, dv);
                 X[active ? me0 + s : dummy] = v;
                 xprev = v;
             }
@@ -500,7 +509,8 @@ __global__ __launch_bounds__(JT) void k_jacobi_block(SweepArgs A, const float* _
     __syncthreads();
 
     const int kind = A.kind;
-    const float a = A.a, denom = A.denom, inv = A.inv_denom;
+    const float a = A.a;
+    const Divisor dv = divisor_of(A);
 #pragma unroll
     for (int s = 1; s <= M; ++s) {
         constexpr int dummy = 0;
@@ -516,7 +526,7 @@ __global__ __launch_bounds__(JT) void k_jacobi_block(SweepArgs A, const float* _
             if (gi < 1 || gi > R - 2 || gj < 1 || gj > R - 2) continue;
             const int o = r * W + c;
             const float v = relax(kind, rhs[(r - 1) * BW + c - 1], cur[o - W], cur[o + W], cur[o - 1], cur[o + 1], a,
-                                  denom, inv);
+                                  dv);
             const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
             const float cn = 0.5f * (tb + lr);
             if (!last) {
@@ -577,21 +587,31 @@ __device__ __forceinline__ void jacobi_strip(const SweepArgs& A, const float* __
     // ---- all loads in flight before the first use
     if (EDGE) {
         const int cj = min(max(gj, 0), R - 1), bj = min(max(gj, 1), R - 2);
+        if (zero_src) {
 #pragma unroll
-        for (int q = 0; q < H; ++q)
-            X[q] = zero_src ? 0.f : src[(size_t)min(max(gi0 + q, 0), R - 1) * P + cj];
+            for (int q = 0; q < H; ++q) X[q] = 0.f;
+        } else {
+#pragma unroll
+            for (int q = 0; q < H; ++q) X[q] = src[(size_t)min(max(gi0 + q, 0), R - 1) * P + cj];
+        }
 #pragma unroll
         for (int q = 1; q < H - 1; ++q) B[q] = A.b[(size_t)min(max(gi0 + q, 1), R - 2) * P + bj];
     } else {
         const size_t o = (size_t)gi0 * P + gj;
+        if (zero_src) {
 #pragma unroll
-        for (int q = 0; q < H; ++q) X[q] = zero_src ? 0.f : src[o + (size_t)q * P];
+            for (int q = 0; q < H; ++q) X[q] = 0.f;
+        } else {
+#pragma unroll
+            for (int q = 0; q < H; ++q) X[q] = src[o + (size_t)q * P];
+        }
 #pragma unroll
         for (int q = 1; q < H - 1; ++q) B[q] = A.b[o + (size_t)q * P];
     }
     B[0] = B[H - 1] = 0.f;
 
-    const float a = A.a, denom = A.denom, inv = A.inv_denom;
+    const float a = A.a;
+    const Divisor dv = divisor_of(A);
     const Ring rg = A.ring;
     const bool left = gj == 0, right = gj == R - 1;
     const bool cols_ring = EDGE && (gj0 <= 0 || gj0 + 63 >= R - 1);  // wave-uniform
@@ -601,7 +621,7 @@ __device__ __forceinline__ void jacobi_strip(const SweepArgs& A, const float* __
 #pragma unroll
         for (int q = s; q < H - s; ++q) {
             const float c = X[q];
-            float v = relax_k<KIND>(B[q], north, X[q + 1], lane_west(c), lane_east(c), a, denom, inv);
+            float v = relax_k<KIND>(B[q], north, X[q + 1], lane_west(c), lane_east(c), a, dv);
             north = c;
             if (EDGE && s < M) {
                 // ring of this sweep: the bottom ring row from the new row above,
@@ -896,6 +916,12 @@ hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a
 // ---- Jacobi ------------------------------------------------------------------
 int jacobi_launches(int K, int block) { return (K + block - 1) / block; }
 
+hipError_t launch_jacobi_pass(hipStream_t s, const SweepArgs& a, const float* src, float* dst, int zero_src, int m,
+                              int kernel) {
+    return kernel == JACOBI_LDS ? launch_jacobi_block(s, a, src, dst, zero_src, m)
+                                : launch_jacobi_wave(s, a, src, dst, zero_src, m);
+}
+
 hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch, int block, int kernel) {
     // Sweep order: x -> scratch -> x -> ...; an odd launch count ends in
     // scratch, and the caller swaps the two buffers.
@@ -905,8 +931,7 @@ hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch
     bool zero = a.zero_start != 0;
     while (done < a.K) {
         const int m = std::min(block, a.K - done);
-        hipError_t e = kernel == JACOBI_LDS ? launch_jacobi_block(s, a, src, dst, zero ? 1 : 0, m)
-                                            : launch_jacobi_wave(s, a, src, dst, zero ? 1 : 0, m);
+        hipError_t e = launch_jacobi_pass(s, a, src, dst, zero ? 1 : 0, m, kernel);
         if (e != hipSuccess) return e;
         zero = false;
         done += m;

@@ -103,3 +103,38 @@ def test_exact_quotient_and_unit_divisor():
         assert markstein(x, Fraction(1), Fraction(1)) == x
     d = np.float32(4.0)
     assert markstein(f32(10.0), Fraction(4), Fraction(1, 4)) == Fraction(5, 2)
+
+
+def division_floor(d: np.float32) -> float:
+    """Host rule for the smallest nonzero |dividend| of the FMA-corrected
+    division (fluid-2d_amd/csrc/fs2d_internal.hpp division_floor)."""
+    if not d <= np.float32(2.0 ** 100):
+        return float("inf")
+    return max(2.0 ** -100, float(np.float32(d) * np.float32(2.0 ** -124)))
+
+
+@pytest.mark.parametrize("d", [np.float32(1.8e7), np.float32(2.0 ** 26 * 1.5), np.float32(2.0 ** 40 + 12345 * 2.0 ** 17),
+                               np.float32(3.3e20), np.float32(2.0 ** 99 * 1.75)], ids=lambda d: f"{float(d):.3g}")
+def test_markstein_exact_from_the_division_floor_for_large_denominators(d):
+    """Large 1 + 4a (huge diffusion rates): from lo = max(2^-100, d 2^-124)
+    upward the quotient stays normal and the correction exact."""
+    D = Fraction(float(d))
+    y = Fraction(float(np.float32(1.0) / d))
+    lo = division_floor(d)
+    rng = random.Random(int(d.view(np.uint32)))
+    e_lo = int(np.floor(np.log2(lo)))
+    for e in list(range(e_lo, e_lo + 6)) + [rng.randrange(e_lo, 100) for _ in range(300)]:
+        for _ in range(20):
+            m = rng.randrange(1 << 23, 1 << 24)
+            x = Fraction(m) * Fraction(2) ** (e - 23)
+            if x < Fraction(lo) or x > Fraction(2) ** 100:
+                continue
+            assert markstein(x, D, y) == rn32(x / D), (float(x), float(d))
+
+
+def test_division_floor_rule():
+    assert division_floor(np.float32(1.0)) == 2.0 ** -100
+    assert division_floor(np.float32(2.0 ** 24)) == 2.0 ** -100
+    assert division_floor(np.float32(2.0 ** 30)) == 2.0 ** -94
+    assert division_floor(np.float32(2.0 ** 101)) == float("inf")
+    assert division_floor(np.float32(np.nan)) == float("inf")

@@ -155,6 +155,45 @@ def test_non_finite_velocity(oracle_lib):
         assert_same_nan(got[f], want[f], f"non-finite field={f}")
 
 
+def extreme_state(R, seed, with_inf=False):
+    """Fields mixing ordinary values with exact zeros (a whole zero band and
+    scattered zeros), +-tiny values below 2^-100, subnormals and values above
+    2^100: the dividends the FMA-corrected division hands to the IEEE one."""
+    rng = np.random.default_rng(seed)
+    st = list(W.random_state(R, seed=seed))
+    for f in range(3):
+        g = st[f]
+        g[R // 3:R // 2, :] = 0.0
+        for frac, mag in ((0.3, 0.0), (0.1, 1e-33), (0.05, 1e-40), (0.02, 1e32)):
+            m = rng.random((R, R)) < frac
+            g[m] = (mag * rng.choice([-1.0, 1.0], size=(R, R))).astype(np.float32)[m]
+        if with_inf:
+            g[R // 2 + 1, R // 4] = np.inf
+    return tuple(st)
+
+
+@pytest.mark.parametrize("order,block,gsb,lag,jk", [(GS, 0, 8, 4, 0), (GS, 0, 1, 0, 0), (JAC, 4, 0, 0, WAVE),
+                                                    (JAC, 8, 0, 0, WAVE), (JAC, 3, 0, 0, LDS)])
+@pytest.mark.parametrize("R", [10, 67, 130])
+@pytest.mark.parametrize("rate", [5e-2, 1e4, 1e30])
+@pytest.mark.parametrize("with_inf", [False, True])
+def test_diffuse_extreme_values(R, rate, with_inf, order, block, gsb, lag, jk, oracle_lib):
+    """Zeros, tiny, subnormal, huge and infinite values, and diffusion rates
+    whose 1 + 4a exceeds 2^24 (division floor above 2^-100) and 2^100 (every
+    nonzero dividend through the IEEE division)."""
+    st = extreme_state(R, seed=R + int(with_inf), with_inf=with_inf)
+    for K in (1, 6):
+        with make(R, st, order=order, block=block, gsb=gsb, lag=lag, jk=jk) as s:
+            for f in range(3):
+                s.diffuse(f, rate, 0.1, K)
+            got = s.download()
+        for f in range(3):
+            want = st[f].copy()
+            oracle_lib.diffuse(want, f, rate, 0.1, K, oracle_order(order))
+            assert_same_nan(got[f], want, f"extreme diffuse R={R} field={f} rate={rate} K={K} inf={with_inf} "
+                                             f"order={order} block={block} jk={jk}")
+
+
 @pytest.mark.parametrize("order,block,gsb,lag,jk", GS_VARIANTS + [(JAC, 2, 0, 0, WAVE), (JAC, 4, 0, 0, WAVE),
                                                                   (JAC, 5, 0, 0, WAVE), (JAC, 8, 0, 0, WAVE),
                                                                   (JAC, 4, 0, 0, LDS)])
@@ -270,6 +309,31 @@ def test_steps_vs_oracle(R, order, block, gsb, lag, jk, oracle_lib):
             got = s.download()
             for f, name in enumerate(("density", "u", "v")):
                 assert_same(got[f], want[f], f"{name} R={R} order={order} block={block} jk={jk} after {target} steps")
+
+
+@pytest.mark.parametrize("jk", [WAVE, LDS])
+def test_jacobi_autotuned_depth(jk, oracle_lib):
+    """jacobi_block = 0 times the depths 1, 2, 4, 8 on the first solve of each
+    kind and keeps the fastest; the state must come out exactly as the
+    oracle's Jacobi order gives it, whatever depth was chosen."""
+    R = 130
+    st = W.vortex_state(R, seed=5, n_vortices=16)
+    rs, us, vs = W.disc_sources(R, radius=8)
+    want = [x.copy() for x in st]
+    with fs2d_amd.Solver(R, diffuse_iterations=20, pressure_iterations=40, order=JAC, jacobi_block=0,
+                         jacobi_kernel=jk) as s:
+        s.upload(*st)
+        s.set_sources(rs, us, vs)
+        s.step_n(3, 1e-4, 1e-5, 0.05)
+        got = s.download()
+        launches = s.launches_per_step()
+    for _ in range(3):
+        oracle_lib.solve(want[0], rs, 1e-4, want[1], want[2], us, vs, 1e-5, 0.05, 20, 40, oracle.JACOBI)
+    for f, name in enumerate(("density", "u", "v")):
+        assert_same(got[f], want[f], f"autotuned Jacobi {name} jk={jk}")
+    # 3 adds, 3 diffuses of ceil(20/m_d), scatter, 2 x (div, ceil(40/m_p), grad), gather
+    possible = {3 + 3 * -(-20 // md) + 1 + 2 * (2 + -(-40 // mp)) + 1 for md in (1, 2, 4, 8) for mp in (1, 2, 4, 8)}
+    assert launches in possible
 
 
 def test_config2_one_step(oracle_lib):
