@@ -15,6 +15,7 @@
 #include <map>
 #include <string>
 #include <utility>
+#include <vector>
 
 using namespace fs2d;
 
@@ -44,6 +45,9 @@ struct fs2d_handle {
     std::map<std::pair<int, int>, int> jacobi_depth;  // (kind, K) -> sweeps per launch, when tuned
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
     bool speed_valid = false;   // max_bits holds max |u|, |v| of the current velocity
+    long long steps_done = 0;
+    struct StepGraph;
+    std::vector<StepGraph*> graphs;
     std::string err;
 
     float*& rho() { return f[F_RHO]; }
@@ -285,6 +289,96 @@ int step(fs2d_handle* h, float diff, float visc, float dt) {
     return project(h, Kp);
 }
 
+}  // namespace
+
+// Everything on the host that one step reads and changes: the field buffer
+// assignment (the step swaps pointers where the reference copies a grid), the
+// two state flags and the step's scalars.  A step captured from one such
+// state replays correctly from that same state, and leaves `post`.
+struct fs2d_handle::StepGraph {
+    float* pre[F_COUNT];
+    float* post[F_COUNT];
+    bool pre_speed, pre_pzero, post_speed, post_pzero;
+    float diff, visc, dt;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+constexpr size_t kMaxStepGraphs = 32;  // the buffer permutation cycles within 12 steps
+
+void drop_graphs(fs2d_handle* h) {
+    for (auto* g : h->graphs) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        delete g;
+    }
+    h->graphs.clear();
+}
+
+bool same_bits(float a, float b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+
+// One step through a replayed graph.  The first step of a handle runs eagerly:
+// it builds the Gauss-Seidel plans and tunes the Jacobi depth, neither of which
+// may happen inside a capture.  After that each distinct pre-step state is
+// captured once (the capture runs the step's host logic, so the pointer swaps
+// and flags come out as the eager step leaves them), instantiated, and
+// replayed whenever the state recurs.
+int step_graph(fs2d_handle* h, float diff, float visc, float dt) {
+    if (h->prm.launch_mode == FS2D_LAUNCH_EAGER || h->steps_done == 0) return step(h, diff, visc, dt);
+    for (auto* g : h->graphs) {
+        if (std::memcmp(g->pre, h->f, sizeof g->pre) != 0 || g->pre_speed != h->speed_valid ||
+            g->pre_pzero != h->pressure_zero || !same_bits(g->diff, diff) || !same_bits(g->visc, visc) ||
+            !same_bits(g->dt, dt))
+            continue;
+        FS2D_HIP(h, hipGraphLaunch(g->exec, h->stream));
+        std::memcpy(h->f, g->post, sizeof g->post);
+        h->speed_valid = g->post_speed;
+        h->pressure_zero = g->post_pzero;
+        return FS2D_OK;
+    }
+    if (h->graphs.size() >= kMaxStepGraphs) drop_graphs(h);
+    auto* g = new fs2d_handle::StepGraph();
+    std::memcpy(g->pre, h->f, sizeof g->pre);
+    g->pre_speed = h->speed_valid;
+    g->pre_pzero = h->pressure_zero;
+    g->diff = diff;
+    g->visc = visc;
+    g->dt = dt;
+    auto restore = [&]() {
+        std::memcpy(h->f, g->pre, sizeof g->pre);
+        h->speed_valid = g->pre_speed;
+        h->pressure_zero = g->pre_pzero;
+    };
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(h, FS2D_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    const int rc = step(h, diff, visc, dt);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(h->stream, &graph);
+    if (rc == FS2D_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != FS2D_OK || e != hipSuccess) {
+        restore();  // nothing the capture recorded has run
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        delete g;
+        return rc != FS2D_OK ? rc : fail(h, FS2D_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
+    }
+    std::memcpy(g->post, h->f, sizeof g->post);
+    g->post_speed = h->speed_valid;
+    g->post_pzero = h->pressure_zero;
+    h->graphs.push_back(g);
+    FS2D_HIP(h, hipGraphLaunch(g->exec, h->stream));
+    return FS2D_OK;
+}
+
+int advance(fs2d_handle* h, float diff, float visc, float dt) {
+    const int rc = step_graph(h, diff, visc, dt);
+    if (rc == FS2D_OK) ++h->steps_done;
+    return rc;
+}
+
 int copy_in(fs2d_handle* h, float* dst, const float* src, hipMemcpyKind kind) {
     const size_t row = sizeof(float) * h->R;
     FS2D_HIP(h, hipMemcpy2DAsync(dst, sizeof(float) * h->P, src, row, row, h->R, kind, h->stream));
@@ -336,6 +430,8 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_tile_cols must be 0, 64 or 128");
     if (prm.gs_block != 0 && prm.gs_block != 1 && prm.gs_block != 2 && prm.gs_block != 4 && prm.gs_block != 8)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_block must be 0, 1, 2, 4 or 8");
+    if (prm.launch_mode != FS2D_LAUNCH_GRAPH && prm.launch_mode != FS2D_LAUNCH_EAGER)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: launch_mode must be 0 (graph) or 1 (eager)");
     if (prm.gs_lag != 0 && prm.gs_lag != 1 && prm.gs_lag != 4 && prm.gs_lag != 8)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_lag must be 0, 1, 4 or 8");
     if (prm.gs_block > 1 && prm.gs_tile_cols == 128)
@@ -381,6 +477,7 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
 int fs2d_hip_destroy(fs2d_handle* h) {
     if (!h) return FS2D_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
     for (auto& kv : h->gs_plans) gs_plan_destroy(&kv.second);
     for (float* p : h->f)
         if (p) (void)hipFree(p);
@@ -436,9 +533,15 @@ int fs2d_hip_set_sources(fs2d_handle* h, const float* rs, const float* us, const
 int fs2d_hip_step_n(fs2d_handle* h, int n, float diff, float visc, float dt) {
     if (!h || n < 0) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_step_n: bad argument");
     for (int i = 0; i < n; ++i) {
-        int rc = step(h, diff, visc, dt);
+        int rc = advance(h, diff, visc, dt);
         if (rc) return rc;
     }
+    return FS2D_OK;
+}
+
+int fs2d_hip_graph_count(fs2d_handle* h, int* count) {
+    if (!h || !count) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_graph_count: bad argument");
+    *count = static_cast<int>(h->graphs.size());
     return FS2D_OK;
 }
 
@@ -486,7 +589,7 @@ int fs2d_hip_solve(fs2d_handle* h, float* rho, const float* rho_src, float diff,
     int rc;
     if ((rc = fs2d_hip_upload(h, rho, u, v))) return rc;
     if ((rc = fs2d_hip_set_sources(h, rho_src, u_src, v_src))) return rc;
-    if ((rc = step(h, diff, visc, dt))) return rc;
+    if ((rc = advance(h, diff, visc, dt))) return rc;
     return fs2d_hip_download(h, rho, u, v);
 }
 

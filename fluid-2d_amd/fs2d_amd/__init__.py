@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libfs2d_hip.so")
 
 GS_LEX, JACOBI = 0, 1
 JACOBI_WAVE, JACOBI_LDS = 1, 2  # fs2d_params.jacobi_kernel
+LAUNCH_GRAPH, LAUNCH_EAGER = 0, 1  # fs2d_params.launch_mode
 DENSITY, VELOCITY_U, VELOCITY_V = 0, 1, 2
 
 _f, _i, _p = ctypes.c_float, ctypes.c_int, ctypes.c_void_p
@@ -31,7 +32,7 @@ class Fs2dError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("diffuse_iterations", _i), ("pressure_iterations", _i), ("sweep_order", _i),
                 ("jacobi_block", _i), ("gs_tile_cols", _i), ("gs_block", _i), ("gs_lag", _i),
-                ("jacobi_kernel", _i)]
+                ("jacobi_kernel", _i), ("launch_mode", _i)]
 
 
 _lib = None
@@ -57,6 +58,7 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_step_n": (_i, [H, _i, _f, _f, _f]),
         "fs2d_hip_download": (_i, [H, _p, _p, _p]),
         "fs2d_hip_sync": (_i, [H]),
+        "fs2d_hip_graph_count": (_i, [H, ctypes.POINTER(_i)]),
         "fs2d_hip_upload_device": (_i, [H, _p, _p, _p]),
         "fs2d_hip_download_device": (_i, [H, _p, _p, _p]),
         "fs2d_hip_stage_add_sources": (_i, [H, _i, _f]),
@@ -95,7 +97,8 @@ class Solver:
 
     def __init__(self, rows: int, cols: int | None = None, diffuse_iterations: int = 20,
                  pressure_iterations: int = 20, order: int = GS_LEX, jacobi_block: int = 0,
-                 gs_tile_cols: int = 0, gs_block: int = 0, gs_lag: int = 0, jacobi_kernel: int = 0):
+                 gs_tile_cols: int = 0, gs_block: int = 0, gs_lag: int = 0, jacobi_kernel: int = 0,
+                 launch_mode: int = LAUNCH_GRAPH):
         L = library()
         cols = rows if cols is None else cols
         prm = Params()
@@ -108,6 +111,7 @@ class Solver:
         prm.gs_block = gs_block
         prm.gs_lag = gs_lag
         prm.jacobi_kernel = jacobi_kernel
+        prm.launch_mode = launch_mode
         h = _p()
         rc = L.fs2d_hip_create(rows, cols, ctypes.byref(prm), ctypes.byref(h))
         if rc != 0:
@@ -153,6 +157,11 @@ class Solver:
 
     def step_n(self, n: int, diffusion: float, viscosity: float, dt: float) -> None:
         self._ck(self._L.fs2d_hip_step_n(self._h, n, diffusion, viscosity, dt))
+
+    def graph_count(self) -> int:
+        n = _i()
+        self._ck(self._L.fs2d_hip_graph_count(self._h, ctypes.byref(n)))
+        return int(n.value)
 
     def sync(self) -> None:
         self._ck(self._L.fs2d_hip_sync(self._h))
@@ -205,6 +214,7 @@ class Solver:
 EXPORTED_SYMBOLS = (
     "fs2d_default_params", "fs2d_hip_create", "fs2d_hip_destroy", "fs2d_hip_solve", "fs2d_hip_upload",
     "fs2d_hip_set_sources", "fs2d_hip_step_n", "fs2d_hip_download", "fs2d_hip_sync",
+    "fs2d_hip_graph_count",
     "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
     "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
     "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FS2D_ABI_VERSION 1
+#define FS2D_ABI_VERSION 2
 
 enum fs2d_status {
     FS2D_OK = 0,
@@ -52,6 +52,12 @@ enum fs2d_jacobi_kernel {
     FS2D_JACOBI_LDS = 2   /* 256-thread blocks, halo exchange through LDS                         */
 };
 
+/* How fs2d_hip_step_n and fs2d_hip_solve issue a step's launches. */
+enum fs2d_launch_mode {
+    FS2D_LAUNCH_GRAPH = 0, /* each step state captured once as a hipGraph, then replayed (default) */
+    FS2D_LAUNCH_EAGER = 1  /* every launch issued one by one                                     */
+};
+
 /* Boundary rule of a field (src/fluid_solver_cpu.cpp:33-83). */
 enum fs2d_field {
     FS2D_FIELD_DENSITY = 0,    /* continuous ring                    */
@@ -68,6 +74,7 @@ typedef struct fs2d_params {
     int gs_block;            /* Gauss-Seidel sweeps per tile (temporal block): 1, 2, 4 or 8; 0 = default */
     int gs_lag;              /* steps each sweep trails the previous one in a tile: 1, 4 or 8; 0 = default */
     int jacobi_kernel;       /* enum fs2d_jacobi_kernel; 0 = default (register strips)                   */
+    int launch_mode;         /* enum fs2d_launch_mode; 0 = graph replay                                  */
 } fs2d_params;
 
 typedef struct fs2d_handle fs2d_handle;
@@ -103,6 +110,8 @@ int fs2d_hip_step_n(fs2d_handle* h, int n, float diffusion_rate, float viscosity
 /* Wait for the handle's stream, then copy the state out (any pointer may be NULL). */
 int fs2d_hip_download(fs2d_handle* h, float* density, float* u, float* v);
 int fs2d_hip_sync(fs2d_handle* h);
+/* Number of step graphs the handle holds (captured, instantiated, replayable). */
+int fs2d_hip_graph_count(fs2d_handle* h, int* count);
 
 /* ---- device-pointer variants (no host staging) --------------------------
  * Arrays are rows*cols floats on the handle's device, row-major, unpadded.

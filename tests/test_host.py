@@ -32,7 +32,8 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, missing
     assert set(syms) == set(fs2d_amd.EXPORTED_SYMBOLS)
-    assert lib.fs2d_hip_abi_version() == 1
+    assert lib.fs2d_hip_abi_version() == 2
+    assert ctypes.sizeof(fs2d_amd.Params) == 9 * 4
 
 
 def test_default_params_are_the_reference_literals():
@@ -52,6 +53,7 @@ def test_default_params_are_the_reference_literals():
     (10, 10, {"gs_block": 3}, "gs_block"),
     (10, 10, {"gs_lag": 2}, "gs_lag"),
     (10, 10, {"jacobi_kernel": 3}, "jacobi_kernel"),
+    (10, 10, {"launch_mode": 2}, "launch_mode"),
 ])
 def test_create_validates_arguments_before_touching_the_device(rows, cols, kw, msg):
     with pytest.raises(fs2d_amd.Fs2dError, match=msg):
