@@ -29,7 +29,7 @@ struct Ring {
     bool neg_tb, neg_lr;
 };
 
-inline Ring ring_of(int mode) {
+__host__ __device__ inline Ring ring_of(int mode) {
     return Ring{mode == BND_OPP_V, mode == BND_OPP_H};
 }
 

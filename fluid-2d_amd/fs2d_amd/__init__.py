@@ -35,6 +35,10 @@ class Params(ctypes.Structure):
                 ("jacobi_kernel", _i), ("launch_mode", _i)]
 
 
+class BatchSim(ctypes.Structure):
+    _fields_ = [("rows", _i), ("iterations", _i), ("dt", _f), ("diffusion_rate", _f), ("viscosity", _f)]
+
+
 _lib = None
 
 
@@ -59,6 +63,16 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_download": (_i, [H, _p, _p, _p]),
         "fs2d_hip_sync": (_i, [H]),
         "fs2d_hip_graph_count": (_i, [H, ctypes.POINTER(_i)]),
+        "fs2d_hip_batch_create": (_i, [_i, ctypes.POINTER(BatchSim), ctypes.POINTER(Params), ctypes.POINTER(H)]),
+        "fs2d_hip_batch_destroy": (_i, [H]),
+        "fs2d_hip_batch_upload": (_i, [H, _i, _p, _p, _p]),
+        "fs2d_hip_batch_set_sources": (_i, [H, _i, _p, _p, _p]),
+        "fs2d_hip_batch_step_n": (_i, [H, _i]),
+        "fs2d_hip_batch_download": (_i, [H, _i, _p, _p, _p]),
+        "fs2d_hip_batch_sync": (_i, [H]),
+        "fs2d_hip_batch_info": (_i, [H, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                     ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_longlong)]),
+        "fs2d_hip_batch_last_error": (ctypes.c_char_p, [H]),
         "fs2d_hip_upload_device": (_i, [H, _p, _p, _p]),
         "fs2d_hip_download_device": (_i, [H, _p, _p, _p]),
         "fs2d_hip_stage_add_sources": (_i, [H, _i, _f]),
@@ -214,10 +228,86 @@ class Solver:
 EXPORTED_SYMBOLS = (
     "fs2d_default_params", "fs2d_hip_create", "fs2d_hip_destroy", "fs2d_hip_solve", "fs2d_hip_upload",
     "fs2d_hip_set_sources", "fs2d_hip_step_n", "fs2d_hip_download", "fs2d_hip_sync",
-    "fs2d_hip_graph_count",
+    "fs2d_hip_graph_count", "fs2d_hip_batch_create", "fs2d_hip_batch_destroy", "fs2d_hip_batch_upload",
+    "fs2d_hip_batch_set_sources", "fs2d_hip_batch_step_n", "fs2d_hip_batch_download", "fs2d_hip_batch_sync",
+    "fs2d_hip_batch_info", "fs2d_hip_batch_last_error",
     "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
     "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
     "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",
     "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_launches_per_step",
     "fs2d_hip_last_error", "fs2d_hip_abi_version",
 )
+
+
+class Batch:
+    """Many independent simulations advanced together (fs2d_hip_batch_*): each
+    stage is one launch over all of them.  ``sims`` is a list of
+    (rows, iterations, dt, diffusion_rate, viscosity); the Jacobi order is
+    the only one batched."""
+
+    def __init__(self, sims, jacobi_block: int = 0, launch_mode: int = LAUNCH_GRAPH, order: int = JACOBI):
+        L = library()
+        arr = (BatchSim * len(sims))()
+        for k, (rows, iters, dt, diff, visc) in enumerate(sims):
+            arr[k] = BatchSim(rows, iters, dt, diff, visc)
+        prm = Params()
+        L.fs2d_default_params(ctypes.byref(prm))
+        prm.sweep_order = order
+        prm.jacobi_block = jacobi_block
+        prm.launch_mode = launch_mode
+        h = _p()
+        rc = L.fs2d_hip_batch_create(len(sims), arr, ctypes.byref(prm), ctypes.byref(h))
+        if rc != 0:
+            raise Fs2dError(f"fs2d_hip_batch_create failed ({rc}): {L.fs2d_hip_batch_last_error(None).decode()}")
+        self._L, self._h = L, h
+        self.rows = [int(x[0]) for x in sims]
+
+    def _ck(self, rc: int) -> None:
+        if rc != 0:
+            raise Fs2dError(f"fs2d batch error {rc}: {self._L.fs2d_hip_batch_last_error(self._h).decode()}")
+
+    def close(self) -> None:
+        if self._h:
+            self._L.fs2d_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, sim: int, rho, u, v) -> None:
+        R = self.rows[sim]
+        self._ck(self._L.fs2d_hip_batch_upload(self._h, sim, _arr(rho, R), _arr(u, R), _arr(v, R)))
+
+    def set_sources(self, sim: int, rho_src=None, u_src=None, v_src=None) -> None:
+        R = self.rows[sim]
+        args = [None if a is None else _arr(a, R) for a in (rho_src, u_src, v_src)]
+        self._ck(self._L.fs2d_hip_batch_set_sources(self._h, sim, *args))
+
+    def step_n(self, n: int) -> None:
+        self._ck(self._L.fs2d_hip_batch_step_n(self._h, n))
+
+    def sync(self) -> None:
+        self._ck(self._L.fs2d_hip_batch_sync(self._h))
+
+    def download(self, sim: int):
+        R = self.rows[sim]
+        out = [np.empty((R, R), np.float32) for _ in range(3)]
+        self._ck(self._L.fs2d_hip_batch_download(self._h, sim, *(o.ctypes.data for o in out)))
+        return out
+
+    def info(self) -> dict:
+        d, t, l, g = _i(), _i(), _i(), _i()
+        c = ctypes.c_longlong()
+        self._ck(self._L.fs2d_hip_batch_info(self._h, ctypes.byref(d), ctypes.byref(t), ctypes.byref(l),
+                                            ctypes.byref(g), ctypes.byref(c)))
+        return {"jacobi_depth": d.value, "tile_rows": t.value, "launches_per_step": l.value, "graphs": g.value,
+                "interior_cells": c.value}

@@ -173,3 +173,26 @@ def random_state(R: int, seed: int, vel_scale: float = 0.05):
     us = (a[4] - 0.5).astype(np.float32)
     vs = (a[5] - 0.5).astype(np.float32)
     return rho, u, v, rs, us, vs
+
+
+BATCH_SIDES = (64, 96, 128, 192, 256, 384, 512)
+
+
+def batch_config(count: int, seed: int, sides=BATCH_SIDES, iterations=(20, 40), dt_range=(0.02, 0.1)):
+    """The batched workload's simulations: side n uniform over `sides`, dt
+    uniform over dt_range, Jacobi count uniform over `iterations`, each drawn
+    from the simulation's own stream.  Returns [(R, K, dt)]."""
+    out = []
+    for s in range(count):
+        rng = SplitMix64(stream_seed(seed, 1000 + s))
+        n = sides[min(int(rng.uniform() * len(sides)), len(sides) - 1)]
+        dt = dt_range[0] + rng.uniform() * (dt_range[1] - dt_range[0])
+        K = iterations[min(int(rng.uniform() * len(iterations)), len(iterations) - 1)]
+        out.append((n + 2, K, float(np.float32(dt))))
+    return out
+
+
+def batch_state(R: int, seed: int, sim_index: int):
+    """A batched simulation's own seeded 16-vortex start (radii 10-40 cells
+    scaled by n / 256)."""
+    return vortex_state(R, seed, n_vortices=16, sim_index=sim_index, reference_n=256)

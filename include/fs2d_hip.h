@@ -153,6 +153,45 @@ int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches);
 const char* fs2d_hip_last_error(const fs2d_handle* h);
 int fs2d_hip_abi_version(void);
 
+/* ---- batched simulations -------------------------------------------------
+ * Many independent simulations of different sizes advanced together: every
+ * stage of the step is one launch over a tile work list spanning all of
+ * them, and the step is replayed as a hipGraph.  No counterpart in the
+ * reference (one fluid_solver per grid, src/simulation.cpp:17-26); each
+ * simulation's results equal those of an fs2d_handle in the Jacobi order
+ * with the same parameters, bit for bit.  Simulations are indexed 0..count-1
+ * in the order given. */
+typedef struct fs2d_batch_sim {
+    int rows;             /* R = n + 2 (square grid, ring included), 3 .. 8194            */
+    int iterations;       /* Jacobi sweeps of each diffuse and pressure solve, >= 2      */
+    float dt;
+    float diffusion_rate; /* density diffusion                                            */
+    float viscosity;      /* velocity diffusion                                           */
+} fs2d_batch_sim;
+
+typedef struct fs2d_batch fs2d_batch;
+
+/* params may be NULL; only sweep_order (must be FS2D_ORDER_JACOBI),
+ * jacobi_block (0 = autotuned; otherwise it must divide every simulation's
+ * iterations at least twice) and launch_mode are read.  Fields start at zero. */
+int fs2d_hip_batch_create(int count, const fs2d_batch_sim* sims, const fs2d_params* params, fs2d_batch** out);
+int fs2d_hip_batch_destroy(fs2d_batch* b);
+/* Host arrays of rows*rows floats for simulation `sim`. */
+int fs2d_hip_batch_upload(fs2d_batch* b, int sim, const float* density, const float* u, const float* v);
+/* NULL = all-zero source. */
+int fs2d_hip_batch_set_sources(fs2d_batch* b, int sim, const float* density_source, const float* u_source,
+                               const float* v_source);
+/* Enqueue n steps of every simulation; returns without waiting. */
+int fs2d_hip_batch_step_n(fs2d_batch* b, int n);
+/* Wait for the batch's stream, then copy one simulation out (any pointer may be NULL). */
+int fs2d_hip_batch_download(fs2d_batch* b, int sim, float* density, float* u, float* v);
+int fs2d_hip_batch_sync(fs2d_batch* b);
+/* The configuration in use: Jacobi sweeps per launch, rows per Jacobi tile,
+ * kernel launches per step, captured step graphs, total interior cells. */
+int fs2d_hip_batch_info(fs2d_batch* b, int* jacobi_depth, int* tile_rows, int* launches_per_step, int* graphs,
+                        long long* interior_cells);
+const char* fs2d_hip_batch_last_error(const fs2d_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

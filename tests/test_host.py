@@ -60,6 +60,31 @@ def test_create_validates_arguments_before_touching_the_device(rows, cols, kw, m
         fs2d_amd.Solver(rows, cols, **kw)
 
 
+@pytest.mark.parametrize("sims,kw,msg", [
+    ([], {}, "count"),
+    ([(2, 8, 0.1, 0.0, 0.0)], {}, "rows"),
+    ([(10, 1, 0.1, 0.0, 0.0)], {}, "iterations"),
+    ([(10, 8, 0.1, 0.0, 0.0)], {"order": fs2d_amd.GS_LEX}, "Jacobi"),
+    ([(10, 8, 0.1, 0.0, 0.0), (12, 6, 0.1, 0.0, 0.0)], {"jacobi_block": 4}, "jacobi_block"),
+    ([(10, 8, 0.1, 0.0, 0.0)], {"jacobi_block": 3}, "jacobi_block"),
+    ([(10, 8, 0.1, 0.0, 0.0)], {"launch_mode": 5}, "launch_mode"),
+])
+def test_batch_create_validates_arguments_before_touching_the_device(sims, kw, msg):
+    with pytest.raises(fs2d_amd.Fs2dError, match=msg):
+        fs2d_amd.Batch(sims, **kw)
+
+
+def test_batch_config_draws_the_stated_distributions():
+    cfg = W.batch_config(256, seed=0)
+    assert len(cfg) == 256
+    sides = {R - 2 for R, _, _ in cfg}
+    assert sides <= set(W.BATCH_SIDES) and len(sides) == len(W.BATCH_SIDES)
+    assert {K for _, K, _ in cfg} == {20, 40}
+    dts = np.array([dt for _, _, dt in cfg])
+    assert dts.min() >= 0.02 and dts.max() <= 0.1 and dts.std() > 0.015
+    assert cfg == W.batch_config(256, seed=0)  # seeded
+
+
 def test_oracle_is_not_linked_into_the_product():
     """The product library must not depend on the oracle (no CPU fallback)."""
     so = open(fs2d_amd.LIB_PATH, "rb").read()
