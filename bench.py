@@ -8,11 +8,16 @@ one fluid_solver::solve (src/fluid_solver_cpu.cpp:6-31) on the resident
 state; inputs are in HBM before the timed region starts.
 
 The headline is the Gauss-Seidel order, whose results equal the reference
-solver's bit for bit.  The Jacobi order (a different relaxation, agreeing with
-the reference only where the solves converge) is reported beside it.
+solver's bit for bit.  Reported beside it: the Jacobi order (a different
+relaxation, agreeing with the reference only where the solves converge) with
+its autotuned temporal-block depth; both orders at 8192 x 8192; and the
+batched workload, 256 independent simulations (sides 64..512, dt in
+[0.02, 0.1], 20 or 40 Jacobi sweeps, seeded 16-vortex starts) advanced by one
+launch per stage.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096]
-                    [--order gs|jacobi] [--no-cpu] [--no-jacobi]
+                    [--order gs|jacobi] [--no-cpu] [--no-jacobi] [--no-8192]
+                    [--no-batch]
 
 Prints one JSON line.
 """
@@ -59,7 +64,7 @@ def bytes_per_cell_step(kd: int, kp: int, a_dens: bool, a_visc: bool) -> int:
     return b
 
 
-def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state, params):
+def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state, params, time_solve=True):
     import fs2d_amd
 
     R = n + 2
@@ -76,10 +81,39 @@ def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state
     # dominant kernel: the pressure relaxation (K_p sweeps), HIP events on the
     # solver's own stream around repeated solves
     reps = max(3, min(20, int(0.5 / max(elapsed / steps / 5, 1e-4))))
-    ms_solve = s.time_pressure_sweeps(reps)
+    ms_solve = s.time_pressure_sweeps(reps) if time_solve else None
     launches = s.launches_per_step()
     s.close()
     return elapsed / steps, ms_solve, launches
+
+
+def run_batch(count: int, steps: int, warmup: int):
+    """The batched workload: `count` simulations drawn by workloads.batch_config
+    (seed 0), diffusion 1e-5, viscosity 1e-6, no sources."""
+    import fs2d_amd
+    from fs2d_amd import workloads as W
+
+    cfg = W.batch_config(count, seed=0)
+    sims = [(R, K, dt, 1e-5, 1e-6) for R, K, dt in cfg]
+    with fs2d_amd.Batch(sims) as b:
+        for k, (R, K, dt) in enumerate(cfg):
+            b.upload(k, *W.batch_state(R, seed=0, sim_index=k))
+            b.set_sources(k)
+        b.step_n(warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step_n(steps)
+        b.sync()
+        sec = (time.perf_counter() - t0) / steps
+        info = b.info()
+    cells = info["interior_cells"]
+    return {"simulations": count, "interior_cells": cells, "steps": steps, "value": round(1.0 / sec, 2),
+            "unit": "steps/s (all simulations)", "ms_per_step": round(sec * 1e3, 3),
+            "cell_updates_per_s": round(cells / sec, 1), "jacobi_depth": info["jacobi_depth"],
+            "tile_rows": info["tile_rows"], "launches_per_step": info["launches_per_step"],
+            "graph_replayed": info["graphs"] > 0,
+            "workload": "sides uniform over {64,96,128,192,256,384,512}, dt ~ U[0.02,0.1], K in {20,40}, "
+                        "16-vortex starts, diffusion 1e-5, viscosity 1e-6, Jacobi order"}
 
 
 def cpu_baseline(n: int, kd: int, kp: int, params, sample_n: int):
@@ -112,12 +146,16 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=3)
+    # a step's buffer rotation repeats within 12 steps: 15 warm-up steps
+    # capture every step graph before the timed region
+    ap.add_argument("--warmup", type=int, default=15)
     ap.add_argument("--n", type=int, default=4096, help="interior cells per side")
     ap.add_argument("--iters", type=int, default=80, help="K_d = K_p")
     ap.add_argument("--order", choices=("gs", "jacobi"), default="gs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
+    ap.add_argument("--no-8192", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=2048, help="interior size of the timed CPU sample")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -175,9 +213,28 @@ def main() -> int:
     if order == 0 and not args.no_jacobi:
         js, jms, jl = run_gpu(1, n, K, K, args.steps, args.warmup, state, params)
         jg = bpc * cells / js / 1e9
-        out["jacobi"] = {"value": round(1.0 / js, 3), "ms_per_step": round(js * 1e3, 3),
+        out["jacobi"] = {"value": round(1.0 / js, 3), "unit": "steps/s", "ms_per_step": round(js * 1e3, 3),
+                         "cell_updates_per_s": round(cells / js, 1),
                          "step_hbm_gbs_algorithmic": round(jg, 1), "step_roofline_frac": round(jg / HBM_PEAK_GBS, 4),
-                         "pressure_solve_ms": round(jms, 4), "launches_per_step": jl}
+                         "pressure_solve_ms": round(jms, 4),
+                         "pressure_solve_gbs_algorithmic": round(12 * cells * K / (jms * 1e-3) / 1e9, 1),
+                         "launches_per_step": jl, "kernel": "k_jacobi_wave (depth autotuned in {1,2,4,8})"}
+    del state
+    if not args.no_8192 and n == 4096:
+        n8, R8 = 8192, 8194
+        st8 = W.vortex_state(R8, seed=3)
+        big = {}
+        for name, o in (("gs", 0), ("jacobi", 1)):
+            if o == 1 and args.no_jacobi:
+                continue
+            sec8, _, _ = run_gpu(o, n8, K, K, 5, max(args.warmup, 13), st8, params, time_solve=False)
+            g8 = bpc * n8 * n8 / sec8 / 1e9
+            big[name] = {"value": round(1.0 / sec8, 3), "unit": "steps/s", "ms_per_step": round(sec8 * 1e3, 3),
+                         "step_hbm_gbs_algorithmic": round(g8, 1), "step_roofline_frac": round(g8 / HBM_PEAK_GBS, 4)}
+        del st8
+        out["n8192"] = big
+    if not args.no_batch:
+        out["batched"] = run_batch(256, max(20, 5 * args.steps), max(args.warmup, 13))
     if not args.no_cpu:
         cb = cpu_baseline(n, K, K, params, min(args.cpu_sample_n, n))
         out["cpu_baseline"] = cb
