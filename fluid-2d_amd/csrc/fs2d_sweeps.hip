@@ -224,9 +224,9 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
 // step s relaxes skewed column j' = TW J' + 1 + s - r.  In these coordinates all
 // M sweeps of a cell's neighbourhood sit on the same anti-diagonal: a cell's
 // sweep-k value is written at step t, read by sweep k + 1 at step t + 1 and
-// overwritten at step t + 2, so the M waves relax in lockstep, in place in LDS,
-// with one barrier per step.  Units with equal I' + J' + kappa touch disjoint
-// cells and are launched together.
+// overwritten at step t + 2, so the M waves relax in place in LDS, each
+// trailing the previous sweep's wave (the lagged schedule below).  Units with
+// equal I' + J' + kappa touch disjoint cells and are launched together.
 template <int TW, int M, int L, int KIND>
 __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __restrict__ work) {
     // blocks [f * units, (f + 1) * units) relax field f of the set, unit by unit
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
         }
     }
 
-    // ---- lockstep sweep
+    // ---- the M sweeps
     const int w = tid >> 6, r = tid & 63;
     const int k = k_lo + w;
     const int i = GS_ROWS * I + 1 + r - k;
