@@ -64,12 +64,15 @@ def bytes_per_cell_step(kd: int, kp: int, a_dens: bool, a_visc: bool) -> int:
     return b
 
 
+LAUNCH_MODE = 0  # fs2d_amd.LAUNCH_GRAPH; --launch eager sets 1
+
+
 def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state, params, time_solve=True):
     import fs2d_amd
 
     R = n + 2
     diff, visc, dt = params
-    s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order)
+    s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order, launch_mode=LAUNCH_MODE)
     s.upload(*state)
     s.set_sources(None, None, None)
     s.step_n(warmup, diff, visc, dt)
@@ -95,7 +98,7 @@ def run_batch(count: int, steps: int, warmup: int):
 
     cfg = W.batch_config(count, seed=0)
     sims = [(R, K, dt, 1e-5, 1e-6) for R, K, dt in cfg]
-    with fs2d_amd.Batch(sims) as b:
+    with fs2d_amd.Batch(sims, launch_mode=LAUNCH_MODE) as b:
         for k, (R, K, dt) in enumerate(cfg):
             b.upload(k, *W.batch_state(R, seed=0, sim_index=k))
             b.set_sources(k)
@@ -156,8 +159,12 @@ def main() -> int:
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--no-8192", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--launch", choices=("graph", "eager"), default="graph",
+                    help="replay captured step graphs (default) or issue every launch")
     ap.add_argument("--cpu-sample-n", type=int, default=2048, help="interior size of the timed CPU sample")
     args = ap.parse_args()
+    global LAUNCH_MODE
+    LAUNCH_MODE = 1 if args.launch == "eager" else 0
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported (a single grid would "
               "shard as row strips with a halo exchange per stage, which is not built)", file=sys.stderr)
@@ -192,7 +199,7 @@ def main() -> int:
         "data": "synthetic (seeded 64-vortex velocity, smoothed-noise density)",
         "config": {"workload": f"stable-fluids step {n}x{n}+ring, dt=0.05, visc=1e-6, diff=1e-5, "
                                f"K_d=K_p={K}, {'Gauss-Seidel (reference order)' if order == 0 else 'Jacobi'}",
-                   "order": args.order, "n": n, "iterations": K},
+                   "order": args.order, "n": n, "iterations": K, "launch": args.launch},
         "cell_updates_per_s": round(cells / sec_step, 1),
         "step_hbm_gbs_algorithmic": round(step_gbs, 1),
         "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),

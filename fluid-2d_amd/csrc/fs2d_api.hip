@@ -300,6 +300,7 @@ struct fs2d_handle::StepGraph {
     float* post[F_COUNT];
     bool pre_speed, pre_pzero, post_speed, post_pzero;
     float diff, visc, dt;
+    hipGraph_t graph = nullptr;  // kept while its executable lives (tools inspect it at launch)
     hipGraphExec_t exec = nullptr;
 };
 
@@ -310,6 +311,7 @@ constexpr size_t kMaxStepGraphs = 32;  // the buffer permutation cycles within 1
 void drop_graphs(fs2d_handle* h) {
     for (auto* g : h->graphs) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
         delete g;
     }
     h->graphs.clear();
@@ -355,13 +357,12 @@ int step_graph(fs2d_handle* h, float diff, float visc, float dt) {
         return fail(h, FS2D_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
     }
     const int rc = step(h, diff, visc, dt);
-    hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(h->stream, &graph);
-    if (rc == FS2D_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-    if (graph) (void)hipGraphDestroy(graph);
+    e = hipStreamEndCapture(h->stream, &g->graph);
+    if (rc == FS2D_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
     if (rc != FS2D_OK || e != hipSuccess) {
         restore();  // nothing the capture recorded has run
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
         delete g;
         return rc != FS2D_OK ? rc : fail(h, FS2D_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
     }

@@ -277,6 +277,7 @@ struct BatchGraph {
     float* pre[A_COUNT];
     float* post[A_COUNT];
     bool pre_speed;
+    hipGraph_t graph = nullptr;  // kept while its executable lives (tools inspect it at launch)
     hipGraphExec_t exec = nullptr;
 };
 
@@ -499,6 +500,7 @@ constexpr size_t kMaxBatchGraphs = 16;
 void drop_graphs(fs2d_batch* b) {
     for (auto* g : b->graphs) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
         delete g;
     }
     b->graphs.clear();
@@ -531,14 +533,13 @@ int advance(fs2d_batch* b) {
         return bfail(b, FS2D_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
     }
     const int rc = step(b);
-    hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(b->stream, &graph);
-    if (rc == FS2D_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-    if (graph) (void)hipGraphDestroy(graph);
+    e = hipStreamEndCapture(b->stream, &g->graph);
+    if (rc == FS2D_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
     if (rc != FS2D_OK || e != hipSuccess) {
         std::memcpy(b->roles.a, g->pre, sizeof g->pre);  // nothing the capture recorded has run
         b->speed_valid = g->pre_speed;
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
         delete g;
         return rc != FS2D_OK ? rc : bfail(b, FS2D_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
     }
