@@ -6,6 +6,7 @@
 // fluid_solver_cpu::solve (src/fluid_solver_cpu.cpp:6-31) and swaps buffer
 // pointers instead of copying where the reference copies a grid
 // (initial_grid in advect, temp_* before the velocity advection).
+#include "../../include/fs2d_cnn.h"
 #include "../../include/fs2d_hip.h"
 #include "fs2d_internal.hpp"
 
@@ -46,6 +47,7 @@ struct fs2d_handle {
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
     bool speed_valid = false;   // max_bits holds max |u|, |v| of the current velocity
     long long steps_done = 0;
+    double* res_sums = nullptr;  // 2 doubles: residual sums of the seeded projection
     struct StepGraph;
     std::vector<StepGraph*> graphs;
     std::string err;
@@ -479,6 +481,7 @@ int fs2d_hip_destroy(fs2d_handle* h) {
     if (!h) return FS2D_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
+    if (h->res_sums) (void)hipFree(h->res_sums);
     for (auto& kv : h->gs_plans) gs_plan_destroy(&kv.second);
     for (float* p : h->f)
         if (p) (void)hipFree(p);
@@ -672,6 +675,88 @@ int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
     // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather
     const bool shared = h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block > 1;
     *launches = 3 + (shared ? d : 3 * d) + 1 + 2 * (2 + p) + 1;
+    return FS2D_OK;
+}
+
+// ---- the seeded projection (include/fs2d_cnn.h) ---------------------------
+static int residual_of(fs2d_handle* h, double* rel) {
+    if (!h->res_sums) FS2D_HIP(h, hipMalloc(&h->res_sums, 2 * sizeof(double)));
+    FS2D_HIP(h, launch_fill(h->stream, h->res_sums, 0u, 4));
+    FS2D_HIP(h, launch_pressure_residual(h->stream, h->f[F_D], h->f[F_P], h->R, h->P, h->res_sums));
+    double sums[2] = {0, 0};
+    FS2D_HIP(h, hipMemcpyAsync(sums, h->res_sums, sizeof sums, hipMemcpyDeviceToHost, h->stream));
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    *rel = sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : std::sqrt(sums[0]);
+    return FS2D_OK;
+}
+
+int fs2d_hip_pressure_residual(fs2d_handle* h, double* rel) {
+    if (!h || !rel) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_pressure_residual: bad argument");
+    return residual_of(h, rel);
+}
+
+// Jacobi launches on the handle's pressure field until the relative residual
+// is <= tol (checked every `check` sweeps) or max_sweeps have run.
+static int jacobi_until(fs2d_handle* h, float tol, int max_sweeps, int check, int* sweeps, double* rel) {
+    const int depth = jacobi_depth_known(h, SWEEP_PRESSURE, h->prm.pressure_iterations);
+    const SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, max_sweeps, 0);
+    int rc;
+    if ((rc = residual_of(h, rel))) return rc;
+    int done = 0, since = 0;
+    while (done < max_sweeps && !(*rel <= tol)) {
+        const int m = std::min(depth, max_sweeps - done);
+        FS2D_HIP(h, launch_jacobi_pass(h->stream, A, h->f[F_P], h->f[F_SCRATCH], 0, m, h->jacobi_kernel));
+        std::swap(h->f[F_P], h->f[F_SCRATCH]);
+        done += m;
+        since += m;
+        if (since >= check || done >= max_sweeps) {
+            since = 0;
+            if ((rc = residual_of(h, rel))) return rc;
+        }
+    }
+    *sweeps = done;
+    return FS2D_OK;
+}
+
+int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, int max_sweeps, int check,
+                            int fallback, int* sweeps_run, double* rel_residual, int* fell_back) {
+    if (!h || seed < 0 || seed > 2 || max_sweeps < 0 || check < 1 || !(tol >= 0.f))
+        return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_project_seeded: bad argument");
+    if (h->prm.sweep_order != FS2D_ORDER_JACOBI)
+        return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_project_seeded: needs the Jacobi order");
+    if (seed == 2 && (!net || (h->R - 2) % 32 != 0))
+        return fail(h, FS2D_ERR_ARGUMENT,
+                    "fs2d_hip_project_seeded: seed 2 needs a network and a side n = rows - 2 that is a multiple of 32");
+    const Scalars s = scalars(h->R, 1.0f);
+    FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h, nullptr));
+    if (seed == 0 || h->pressure_zero) FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
+    const bool keep_prev = seed == 2 && fallback;
+    if (keep_prev)  // the previous pressure, for the fallback
+        FS2D_HIP(h, hipMemcpyAsync(h->f[F_T1], h->f[F_P], h->field_bytes, hipMemcpyDeviceToDevice, h->stream));
+    if (seed == 2) {
+        const int rc = net_predict_fields(net, h->stream, h->f[F_D], h->f[F_P], h->R, h->P, h->f[F_SCRATCH]);
+        if (rc) return fail(h, rc, std::string("fs2d_hip_project_seeded: network: ") + net_error(net));
+        std::swap(h->f[F_P], h->f[F_SCRATCH]);
+    }
+    int sweeps = 0, rc;
+    double rel = 0;
+    if ((rc = jacobi_until(h, tol, max_sweeps, check, &sweeps, &rel))) return rc;
+    int fb = 0;
+    if (keep_prev && !(rel <= tol)) {
+        FS2D_HIP(h, hipMemcpyAsync(h->f[F_P], h->f[F_T1], h->field_bytes, hipMemcpyDeviceToDevice, h->stream));
+        int more = 0;
+        if ((rc = jacobi_until(h, tol, max_sweeps, check, &more, &rel))) return rc;
+        sweeps += more;
+        fb = 1;
+    }
+    h->pressure_zero = false;
+    FS2D_HIP(h, launch_gradient(h->stream, h->u(), h->v(), h->f[F_P], h->R, h->P, s.h, 0, h->max_bits,
+                                h->speed_slots));
+    h->speed_valid = true;
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    if (sweeps_run) *sweeps_run = sweeps;
+    if (rel_residual) *rel_residual = rel;
+    if (fell_back) *fell_back = fb;
     return FS2D_OK;
 }
 

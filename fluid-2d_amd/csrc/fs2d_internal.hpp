@@ -91,6 +91,11 @@ hipError_t launch_gather_velocity(hipStream_t s, float* u1, float* v1, const flo
 // p_zero (may be null): also zero this pressure iterate, ring included.
 hipError_t launch_divergence(hipStream_t s, float* d, const float* u, const float* v, int R, int P,
                              float neg_half_h, float* p_zero);
+// sums[0] += sum ||d - L p||^2 and sums[1] += sum d^2 over the interior (FP64),
+// L the compact 5-point operator 4 p - (pN + pS + pW + pE) of the pressure
+// sweeps (src/fluid_solver_cpu.cpp:196-204) with p's stored ring.
+hipError_t launch_pressure_residual(hipStream_t s, const float* d, const float* p, int R, int P, double* sums);
+
 // Max-speed reduction slots: kSpeedSlots words, kSlotStride words apart
 // (separate cache lines), zero between uses.
 constexpr int kSpeedSlots = 256, kSlotStride = 32;
@@ -131,4 +136,12 @@ hipError_t launch_jacobi_pass(hipStream_t s, const SweepArgs& a, const float* sr
                               int kernel);
 int jacobi_launches(int K, int block);
 
+}  // namespace fs2d
+
+struct fs2d_net;
+namespace fs2d {
+// The pressure network's prediction from pitched fields d and prev into p
+// (fs2d_cnn.hip), enqueued on `stream`.
+int net_predict_fields(::fs2d_net* net, hipStream_t stream, const float* d, const float* prev, int R, int P, float* p);
+const char* net_error(const ::fs2d_net* net);
 }  // namespace fs2d

