@@ -67,6 +67,13 @@ def bytes_per_cell_step(kd: int, kp: int, a_dens: bool, a_visc: bool) -> int:
 LAUNCH_MODE = 0  # fs2d_amd.LAUNCH_GRAPH; --launch eager sets 1
 
 
+def fs2d_amd_copy_bandwidth() -> float:
+    """On-box copy bandwidth (1 GiB float4 copy kernel, read + write)."""
+    import fs2d_amd
+
+    return fs2d_amd.copy_bandwidth(1 << 30, 20)
+
+
 def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state, params, time_solve=True):
     import fs2d_amd
 
@@ -119,6 +126,38 @@ def run_batch(count: int, steps: int, warmup: int):
                         "16-vortex starts, diffusion 1e-5, viscosity 1e-6, Jacobi order"}
 
 
+def run_learned_guess():
+    """Config 5: projections solved to ||d - L p|| <= 1e-4 ||d|| seeded from
+    the previous pressure (plain Jacobi) or from the network, at 1024^2 and
+    2048^2, on the same seeded 64-vortex flow (steps 100..), per projection."""
+    import fs2d_amd  # noqa: F401  (library first)
+    from fs2d_amd import pressure_net as PN
+
+    if not os.path.exists(PN.DEFAULT_PARAMS):
+        return {"status": "no trained parameters; run PYTHONPATH=fluid-2d_amd python -m fs2d_amd.train_pressure_net"}
+    net = PN.PressureNet(seed=0)
+    net.load(PN.DEFAULT_PARAMS)
+    out = {"tolerance": "||d - L p||_2 <= 1e-4 ||d||_2 (compact Poisson residual)",
+           "method": "every projection of 3 (1024^2) / 2 (2048^2) steps solved to the tolerance after one warm-up step"}
+    for R, steps in ((1026, 3), (2050, 2)):
+        r = PN.projection_race(R, net, seed=7, start_step=100, steps=steps)
+        plain, learned = r[1], r[2]
+        out[f"n{R - 2}"] = {
+            "plain_jacobi_from_previous": {"ms_per_projection": round(plain["seconds"] * 1e3, 3),
+                                           "sweeps_per_projection": round(plain["sweeps"], 1)},
+            "learned_guess": {"ms_per_projection": round(learned["seconds"] * 1e3, 3),
+                              "sweeps_per_projection": round(learned["sweeps"], 1),
+                              "fallbacks": learned["fallbacks"], "projections": learned["projections"]},
+            "speedup": round(plain["seconds"] / max(learned["seconds"], 1e-9), 3)}
+    ms = net.time_forward(1, 1024, 1024, reps=10)
+    fl = net.forward_flops(1, 1024, 1024)
+    out["network_forward_1024"] = {"ms": round(ms, 4), "tflops": round(fl / (ms * 1e-3) / 1e12, 2),
+                                   "mfma_frac_bf16_dense": round(fl / (ms * 1e-3) / 2.5e15, 5),
+                                   "flops": fl}
+    net.close()
+    return out
+
+
 def cpu_baseline(n: int, kd: int, kp: int, params, sample_n: int):
     """The oracle (oracle/fs2d_oracle.c, a bit-exact restatement of the
     reference CPU solver, single-threaded like the reference) timed on one full
@@ -139,7 +178,16 @@ def cpu_baseline(n: int, kd: int, kp: int, params, sample_n: int):
     O.solve(rho, z, diff, u, v, z, z, visc, dt, kd, kp, oracle.GS_LEX)
     sec = time.perf_counter() - t0
     scale = (n * n) / float(sample_n * sample_n)
+    cpu_model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": round(1.0 / (sec * scale), 6), "unit": "steps/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model, "host_cpus": os.cpu_count(),
             "sample": f"one full step at {sample_n}x{sample_n}+ring (K_d=K_p={kd}, same dt/viscosity/diffusion, "
                       f"seeded 64-vortex start) took {sec:.2f} s on one host thread; scaled x{scale:g} by cell "
                       f"count to {n}x{n}; oracle/fs2d_oracle.c, bit-identical to the reference CPU solver"}
@@ -159,6 +207,7 @@ def main() -> int:
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--no-8192", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--launch", choices=("graph", "eager"), default="graph",
                     help="replay captured step graphs (default) or issue every launch")
     ap.add_argument("--cpu-sample-n", type=int, default=2048, help="interior size of the timed CPU sample")
@@ -205,6 +254,7 @@ def main() -> int:
         "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "bytes_per_cell_step": bpc,
         "launches_per_step": launches,
+        "stream_copy_gbs": round(fs2d_amd_copy_bandwidth(), 1),
         "roofline": {
             "kernel": "k_gs_unit<64,8,4,pressure>" if order == 0 else "k_jacobi_block<4>",
             "bound": "hbm",
@@ -242,6 +292,8 @@ def main() -> int:
         out["n8192"] = big
     if not args.no_batch:
         out["batched"] = run_batch(256, max(20, 5 * args.steps), max(args.warmup, 13))
+    if not args.no_learned:
+        out["learned_guess"] = run_learned_guess()
     if not args.no_cpu:
         cb = cpu_baseline(n, K, K, params, min(args.cpu_sample_n, n))
         out["cpu_baseline"] = cb

@@ -660,6 +660,35 @@ int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
     return FS2D_OK;
 }
 
+int fs2d_hip_copy_bandwidth(long long bytes, int reps, float* gbs) {
+    if (bytes < 1024 || reps < 1 || !gbs) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_copy_bandwidth: bad argument");
+    const size_t n4 = static_cast<size_t>(bytes) / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, n4 * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, n4 * 16);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_fill(st, a, 0x3f800000u, n4 * 4);
+    if (e == hipSuccess) e = launch_copy4(st, b, a, n4);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy4(st, (r & 1) ? a : b, (r & 1) ? b : a, n4);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(nullptr, FS2D_ERR_HIP, std::string("fs2d_hip_copy_bandwidth: ") + hipGetErrorString(e));
+    *gbs = static_cast<float>(2.0 * n4 * 16 * reps / (ms * 1e-3) / 1e9);
+    return FS2D_OK;
+}
+
 int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
     if (!h || !launches) return fail(h, FS2D_ERR_ARGUMENT, "launches_per_step: bad argument");
     const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;

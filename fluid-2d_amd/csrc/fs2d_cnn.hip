@@ -663,6 +663,43 @@ int fs2d_net_predict_host(fs2d_net* net, int n, int h, int w, const float* d, co
     return FS2D_OK;
 }
 
+int fs2d_net_time_forward(fs2d_net* net, int n, int h, int w, int reps, float* ms) {
+    int rc;
+    if ((rc = check_shape(net, n, h, w))) return rc;
+    if (reps < 1 || !ms) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_time_forward: bad argument");
+    auto it = net->acts.find(std::make_tuple(n, h, w));
+    if (it == net->acts.end()) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_time_forward: no activations of that shape");
+    hipEvent_t e0, e1;
+    NHIP(net, hipEventCreate(&e0));
+    NHIP(net, hipEventCreate(&e1));
+    if ((rc = forward(net, *it->second))) return rc;  // warm
+    NHIP(net, hipEventRecord(e0, net->stream));
+    for (int r = 0; r < reps && rc == FS2D_OK; ++r) rc = forward(net, *it->second);
+    NHIP(net, hipEventRecord(e1, net->stream));
+    NHIP(net, hipEventSynchronize(e1));
+    float t = 0.f;
+    NHIP(net, hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *ms = t / reps;
+    return FS2D_OK;
+}
+
+int fs2d_net_forward_flops(fs2d_net* net, int n, int h, int w, double* flops) {
+    int rc;
+    if ((rc = check_shape(net, n, h, w))) return rc;
+    if (!flops) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_forward_flops: bad argument");
+    // pixels of each layer's level: e1*, d1*, out at 1; e2*, d2* at 1/4; e3*, d3* at 1/16; e4* at 1/64
+    const int level[kNumLayers] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    double f = 0;
+    const double p1 = static_cast<double>(n) * h * w;
+    for (int l = 0; l < kNumLayers; ++l)
+        f += 2.0 * 9.0 * kLayers[l].cin * kLayers[l].cout * p1 / static_cast<double>(1 << (2 * level[l]));
+    *flops = f;
+    return FS2D_OK;
+}
+
 void* fs2d_net_stream(fs2d_net* net) { return net ? static_cast<void*>(net->stream) : nullptr; }
 
 const char* fs2d_net_last_error(const fs2d_net* net) { return net_error(net); }

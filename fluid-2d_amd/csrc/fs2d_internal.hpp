@@ -105,6 +105,7 @@ constexpr int kSpeedSlots = 256, kSlotStride = 32;
 hipError_t launch_gradient(hipStream_t s, float* u, float* v, const float* p, int R, int P, float h,
                            int p_is_zero, unsigned* max_bits, unsigned* speed_slots);
 hipError_t launch_set_boundary(hipStream_t s, float* x, int R, int P, Ring ring);
+hipError_t launch_copy4(hipStream_t s, float4* dst, const float4* src, size_t n4);
 hipError_t launch_fill(hipStream_t s, void* p, uint32_t value, size_t n_words);
 
 // Gauss-Seidel tile wavefront: one launch per level of the tile hyperplane.

@@ -149,6 +149,10 @@ __global__ __launch_bounds__(256) void k_pressure_residual(const float* __restri
     }
 }
 
+__global__ __launch_bounds__(256) void k_copy4(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256) dst[t] = src[t];
+}
+
 __global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint32_t value, size_t n) {
     for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256) p[t] = value;
 }
@@ -206,6 +210,12 @@ hipError_t launch_pressure_residual(hipStream_t s, const float* d, const float* 
     const long cells = static_cast<long>(R - 2) * (R - 2);
     const unsigned blocks = static_cast<unsigned>(std::max<long>(1, std::min<long>((cells + 255) / 256, 2048)));
     hipLaunchKernelGGL(k_pressure_residual, dim3(blocks), dim3(256), 0, s, d, p, R, P, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy4(hipStream_t s, float4* dst, const float4* src, size_t n4) {
+    const size_t blocks = std::min<size_t>((n4 + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_copy4, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s, dst, src, n4);
     return hipGetLastError();
 }
 

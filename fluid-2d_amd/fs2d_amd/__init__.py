@@ -63,6 +63,7 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_download": (_i, [H, _p, _p, _p]),
         "fs2d_hip_sync": (_i, [H]),
         "fs2d_hip_graph_count": (_i, [H, ctypes.POINTER(_i)]),
+        "fs2d_hip_copy_bandwidth": (_i, [ctypes.c_longlong, _i, ctypes.POINTER(_f)]),
         "fs2d_hip_batch_create": (_i, [_i, ctypes.POINTER(BatchSim), ctypes.POINTER(Params), ctypes.POINTER(H)]),
         "fs2d_hip_batch_destroy": (_i, [H]),
         "fs2d_hip_batch_upload": (_i, [H, _i, _p, _p, _p]),
@@ -228,11 +229,12 @@ class Solver:
 EXPORTED_SYMBOLS = (
     "fs2d_default_params", "fs2d_hip_create", "fs2d_hip_destroy", "fs2d_hip_solve", "fs2d_hip_upload",
     "fs2d_hip_set_sources", "fs2d_hip_step_n", "fs2d_hip_download", "fs2d_hip_sync",
-    "fs2d_hip_graph_count", "fs2d_hip_batch_create", "fs2d_hip_batch_destroy", "fs2d_hip_batch_upload",
+    "fs2d_hip_graph_count", "fs2d_hip_copy_bandwidth", "fs2d_hip_batch_create", "fs2d_hip_batch_destroy", "fs2d_hip_batch_upload",
     "fs2d_hip_batch_set_sources", "fs2d_hip_batch_step_n", "fs2d_hip_batch_download", "fs2d_hip_batch_sync",
     "fs2d_hip_batch_info", "fs2d_hip_batch_last_error",
     "fs2d_net_create", "fs2d_net_destroy", "fs2d_net_param_count", "fs2d_net_get_params", "fs2d_net_set_params",
     "fs2d_net_get_grads", "fs2d_net_train_step", "fs2d_net_predict_host", "fs2d_net_stream", "fs2d_net_last_error",
+    "fs2d_net_time_forward", "fs2d_net_forward_flops",
     "fs2d_hip_project_seeded", "fs2d_hip_pressure_residual",
     "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
     "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
@@ -314,3 +316,13 @@ class Batch:
                                             ctypes.byref(g), ctypes.byref(c)))
         return {"jacobi_depth": d.value, "tile_rows": t.value, "launches_per_step": l.value, "graphs": g.value,
                 "interior_cells": c.value}
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """GB/s of an on-device float4 copy (read + write counted)."""
+    L = library()
+    g = _f()
+    rc = L.fs2d_hip_copy_bandwidth(nbytes, reps, ctypes.byref(g))
+    if rc != 0:
+        raise Fs2dError(f"fs2d_hip_copy_bandwidth failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
+    return float(g.value)

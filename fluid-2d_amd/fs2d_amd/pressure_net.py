@@ -22,6 +22,10 @@ from . import workloads as W
 
 _i, _f, _p = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
 
+# parameters written by python -m fs2d_amd.train_pressure_net
+DEFAULT_PARAMS = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)),
+                                            "pressure_net_params.npz")
+
 LAYERS = [(16, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128),
           (192, 64), (64, 64), (96, 32), (32, 32), (48, 16), (16, 16), (16, 16)]  # (cin, cout), padded
 
@@ -40,6 +44,8 @@ def _bind():
         "fs2d_net_train_step": (_i, [_p, _i, _i, _i, _p, _p, _p, _f, ctypes.POINTER(_f)]),
         "fs2d_net_predict_host": (_i, [_p, _i, _i, _i, _p, _p, _p]),
         "fs2d_net_stream": (_p, [_p]),
+        "fs2d_net_time_forward": (_i, [_p, _i, _i, _i, _i, ctypes.POINTER(_f)]),
+        "fs2d_net_forward_flops": (_i, [_p, _i, _i, _i, ctypes.POINTER(ctypes.c_double)]),
         "fs2d_net_last_error": (ctypes.c_char_p, [_p]),
         "fs2d_hip_project_seeded": (_i, [_p, _p, _i, _f, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(_i)]),
@@ -127,6 +133,20 @@ class PressureNet:
         self._ck(self._L.fs2d_net_predict_host(self._h, n, h, w, d.ctypes.data, prev.ctypes.data, out.ctypes.data))
         return out
 
+    def time_forward(self, n: int, h: int, w: int, reps: int = 10) -> float:
+        """Milliseconds per forward pass (HIP events); runs a prediction of
+        that shape first so its activations exist."""
+        z = np.zeros((n, h, w), np.float32)
+        self.predict(z + 1e-3, z + 1e-3)
+        ms = _f()
+        self._ck(self._L.fs2d_net_time_forward(self._h, n, h, w, reps, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def forward_flops(self, n: int, h: int, w: int) -> float:
+        f = ctypes.c_double()
+        self._ck(self._L.fs2d_net_forward_flops(self._h, n, h, w, ctypes.byref(f)))
+        return float(f.value)
+
     def save(self, path: str) -> None:
         np.savez(path, params=self.params())
 
@@ -178,33 +198,60 @@ def compact_residual(d: np.ndarray, p: np.ndarray) -> float:
     return float(np.linalg.norm(d - Lp) / np.linalg.norm(d))
 
 
-def make_pairs(R: int, seeds, steps, crop: int, crops_per_state: int, rng_seed: int = 0):
+def projection_inputs(solver: Solver, k_from: int, k_to: int, K: int, diff: float, visc: float, dt: float):
+    """Advance the solver (Jacobi order, no sources) from step k_from to k_to
+    stage by stage, as fluid_solver_cpu::solve orders them
+    (src/fluid_solver_cpu.cpp:16-30), yielding (step, projection, d) with d
+    the divergence of the velocity each projection receives."""
+    for k in range(k_from + 1, k_to + 1):
+        solver.diffuse(0, diff, dt, K)
+        solver.advect_density(dt)
+        solver.diffuse(1, visc, dt, K)
+        solver.diffuse(2, visc, dt, K)
+        _, u, v = solver.download()
+        yield k, 1, divergence(u, v)
+        solver.project(K)
+        solver.advect_velocity(dt)
+        _, u, v = solver.download()
+        yield k, 2, divergence(u, v)
+        solver.project(K)
+
+
+def make_pairs(R: int, seeds, steps, crop: int, crops_per_pair: int, K: int = 40, rng_seed: int = 0, log=None):
     """Training crops from seeded 64-vortex runs of the GPU solver (Jacobi
-    order, K = 40): at each step in `steps` the divergence of the velocity is
-    taken, its pressure solved exactly (poisson_dct), and random crop x crop
-    windows of (divergence, previous step's pressure, pressure) kept."""
+    order, K sweeps, dt 0.05, diffusion 1e-5, viscosity 1e-6).  At each step k
+    in `steps` the divergence entering each of its two projections is taken
+    and its pressure solved exactly (poisson_dct); a pair is (divergence, the
+    previous projection's converged pressure, converged pressure), and
+    `crops_per_pair` random crop x crop windows of it are kept."""
     rng = np.random.default_rng(rng_seed)
     n = R - 2
     D, PV, T = [], [], []
     for seed in seeds:
         st = W.vortex_state(R, seed=seed)
-        with Solver(R, diffuse_iterations=40, pressure_iterations=40, order=JACOBI) as s:
+        with Solver(R, diffuse_iterations=K, pressure_iterations=K, order=JACOBI) as s:
             s.upload(*st)
             s.set_sources(None, None, None)
-            done, prev = 0, None
-            for k in sorted(set(steps) | {max(steps[0] - 1, 0)}):
-                s.step_n(k - done, 1e-5, 1e-6, 0.05)
+            done = 0
+            for k in steps:
+                if k - 2 > done:
+                    s.step_n(k - 2 - done, 1e-5, 1e-6, 0.05)
+                    done = k - 2
+                prev = None
+                for kk, which, d in projection_inputs(s, done, k, K, 1e-5, 1e-6, 0.05):
+                    if kk < k and which == 1:
+                        continue  # only step k-1's second projection seeds a pair
+                    p = poisson_dct(d)
+                    if kk == k and prev is not None:
+                        for _ in range(crops_per_pair):
+                            i, j = rng.integers(0, n - crop + 1, size=2)
+                            D.append(d[i:i + crop, j:j + crop].astype(np.float32))
+                            PV.append(prev[i:i + crop, j:j + crop].astype(np.float32))
+                            T.append(p[i:i + crop, j:j + crop].astype(np.float32))
+                    prev = p
                 done = k
-                _, u, v = s.download()
-                d = divergence(u, v)
-                p = poisson_dct(d)
-                if prev is not None and k in steps:
-                    for _ in range(crops_per_state):
-                        i, j = rng.integers(0, n - crop + 1, size=2)
-                        D.append(d[i:i + crop, j:j + crop])
-                        PV.append(prev[i:i + crop, j:j + crop])
-                        T.append(p[i:i + crop, j:j + crop])
-                prev = p
+        if log:
+            log(f"seed {seed}: {len(D)} crops")
     return (np.asarray(D, np.float32), np.asarray(PV, np.float32), np.asarray(T, np.float32))
 
 
@@ -251,3 +298,44 @@ def pressure_residual(solver: Solver) -> float:
     if rc != 0:
         raise Fs2dError(f"fs2d_hip_pressure_residual failed ({rc}): {L.fs2d_hip_last_error(solver._h).decode()}")
     return float(rel.value)
+
+
+def projection_race(R: int, net, seed: int, start_step: int, steps: int, tol: float = 1e-4, max_sweeps: int = 400000,
+                    check: int = 256, K: int = 40, methods=(1, 2)):
+    """Config-5 comparison: from the seeded 64-vortex state at `start_step`
+    (plain steps), one warm-up step whose projections are solved from zero to
+    `tol`, then `steps` steps whose every projection is solved to `tol` from
+    the previous pressure (method 1) or from the network's guess (method 2,
+    with the fallback).  Each method runs on its own handle from the same
+    state.  Returns {method: {"sweeps", "seconds", "fallbacks", "projections",
+    "max_rel"}} with per-projection means."""
+    st = W.vortex_state(R, seed=seed)
+    out = {}
+    for m in methods:
+        with Solver(R, diffuse_iterations=K, pressure_iterations=K, order=JACOBI) as s:
+            s.upload(*st)
+            s.set_sources(None, None, None)
+            s.step_n(start_step, 1e-5, 1e-6, 0.05)
+            rec = {"sweeps": 0, "seconds": 0.0, "fallbacks": 0, "projections": 0, "max_rel": 0.0}
+            for k in range(steps + 1):
+                seed_k = 0 if k == 0 else m
+                for proj in (1, 2):
+                    if proj == 1:
+                        s.diffuse(0, 1e-5, 0.05, K)
+                        s.advect_density(0.05)
+                        s.diffuse(1, 1e-6, 0.05, K)
+                        s.diffuse(2, 1e-6, 0.05, K)
+                    else:
+                        s.advect_velocity(0.05)
+                    sw, rel, fb, sec = seeded_projection(s, net if seed_k == 2 else None, seed_k, tol, max_sweeps,
+                                                         check)
+                    if k > 0:
+                        rec["sweeps"] += sw
+                        rec["seconds"] += sec
+                        rec["fallbacks"] += int(fb)
+                        rec["projections"] += 1
+                        rec["max_rel"] = max(rec["max_rel"], rel)
+            n = max(rec["projections"], 1)
+            out[m] = {"sweeps": rec["sweeps"] / n, "seconds": rec["seconds"] / n, "fallbacks": rec["fallbacks"],
+                      "projections": rec["projections"], "max_rel": rec["max_rel"]}
+    return out

@@ -53,6 +53,14 @@ int fs2d_net_train_step(fs2d_net* net, int n, int h, int w, const float* diverge
 int fs2d_net_predict_host(fs2d_net* net, int n, int h, int w, const float* divergence, const float* prev_pressure,
                           float* out);
 
+/* Mean time of one forward pass over n samples of h x w (the activations of
+ * that shape must exist: run a prediction or training step of it first),
+ * measured with HIP events on the network's stream over `reps` passes. */
+int fs2d_net_time_forward(fs2d_net* net, int n, int h, int w, int reps, float* ms);
+/* Multiply-add FLOPs (2 per MAC, padded channels included: what the MFMA
+ * units execute) of one forward pass over n samples of h x w. */
+int fs2d_net_forward_flops(fs2d_net* net, int n, int h, int w, double* flops);
+
 /* The network's stream (a hipStream_t). */
 void* fs2d_net_stream(fs2d_net* net);
 const char* fs2d_net_last_error(const fs2d_net* net);

@@ -41,8 +41,8 @@ enum fs2d_sweep_order {
      * the step, equal the reference's bit for bit.  Default. */
     FS2D_ORDER_GS_LEX = 0,
     /* Jacobi (every neighbour from the previous iterate), temporally blocked
-     * in LDS.  A different relaxation than the reference's: it agrees with
-     * the reference only where the solve has converged. */
+     * (fs2d_jacobi_kernel).  A different relaxation than the reference's: it
+     * agrees with the reference only where the solve has converged. */
     FS2D_ORDER_JACOBI = 1
 };
 
@@ -146,6 +146,10 @@ void* fs2d_hip_stream(fs2d_handle* h);
  * milliseconds, measured with HIP events on the handle's stream.  The state's
  * velocity is not modified. */
 int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms_per_solve);
+/* On-device copy bandwidth: a float4 copy kernel over `bytes` (read + write
+ * counted), timed with HIP events over `reps` copies; the achievable HBM
+ * figure beside the 8 TB/s peak. */
+int fs2d_hip_copy_bandwidth(long long bytes, int reps, float* gb_per_s);
 /* Number of kernel launches one step enqueues (for per-launch averages). */
 int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches);
 
