@@ -127,9 +127,12 @@ def run_batch(count: int, steps: int, warmup: int):
 
 
 def run_learned_guess():
-    """Config 5: projections solved to ||d - L p|| <= 1e-4 ||d|| seeded from
-    the previous pressure (plain Jacobi) or from the network, at 1024^2 and
-    2048^2, on the same seeded 64-vortex flow (steps 100..), per projection."""
+    """Config 5, measured at equal residual: the first projection of a step of
+    the seeded 64-vortex flow, from identical states, solved by plain Jacobi
+    from the previous pressure for a fixed sweep budget, and by Jacobi from the
+    network's guess until it reaches the same compact residual
+    ||d - L p|| / ||d||; zero start beside them.  Plus the network's forward
+    time and MFMA utilisation at 1024^2."""
     import fs2d_amd  # noqa: F401  (library first)
     from fs2d_amd import pressure_net as PN
 
@@ -137,23 +140,23 @@ def run_learned_guess():
         return {"status": "no trained parameters; run PYTHONPATH=fluid-2d_amd python -m fs2d_amd.train_pressure_net"}
     net = PN.PressureNet(seed=0)
     net.load(PN.DEFAULT_PARAMS)
-    out = {"tolerance": "||d - L p||_2 <= 1e-4 ||d||_2 (compact Poisson residual)",
-           "method": "every projection of 3 (1024^2) / 2 (2048^2) steps solved to the tolerance after one warm-up step"}
-    for R, steps in ((1026, 3), (2050, 2)):
-        r = PN.projection_race(R, net, seed=7, start_step=100, steps=steps)
-        plain, learned = r[1], r[2]
+    out = {"criterion": "compact Poisson residual ||d - L p||_2 / ||d||_2",
+           "target_1e-4": "not reached by plain Jacobi within 131072 sweeps at 1024^2 (0.046), so the "
+                          "comparison is at the residual plain Jacobi reaches in the budget"}
+    for R, budget in ((1026, 16384), (2050, 16384)):
+        r = PN.equal_residual_race(R, net, budget=budget)
+        plain, learned = r["previous_pressure"], r["network"]
         out[f"n{R - 2}"] = {
-            "plain_jacobi_from_previous": {"ms_per_projection": round(plain["seconds"] * 1e3, 3),
-                                           "sweeps_per_projection": round(plain["sweeps"], 1)},
-            "learned_guess": {"ms_per_projection": round(learned["seconds"] * 1e3, 3),
-                              "sweeps_per_projection": round(learned["sweeps"], 1),
-                              "fallbacks": learned["fallbacks"], "projections": learned["projections"]},
-            "speedup": round(plain["seconds"] / max(learned["seconds"], 1e-9), 3)}
+            "budget_sweeps": budget, "target_rel_residual": round(r["target_rel_residual"], 6),
+            "zero_start": {"rel_residual": round(r["zero"]["rel_residual"], 6), "ms": round(r["zero"]["ms"], 3)},
+            "plain_jacobi_from_previous": {"sweeps": plain["sweeps"], "ms": round(plain["ms"], 3)},
+            "learned_guess": {"sweeps": learned["sweeps"], "ms": round(learned["ms"], 3),
+                              "reached": learned["reached"], "rel_residual": round(learned["rel_residual"], 6)},
+            "speedup": round(plain["ms"] / max(learned["ms"], 1e-9), 3)}
     ms = net.time_forward(1, 1024, 1024, reps=10)
     fl = net.forward_flops(1, 1024, 1024)
     out["network_forward_1024"] = {"ms": round(ms, 4), "tflops": round(fl / (ms * 1e-3) / 1e12, 2),
-                                   "mfma_frac_bf16_dense": round(fl / (ms * 1e-3) / 2.5e15, 5),
-                                   "flops": fl}
+                                   "mfma_frac_bf16_dense": round(fl / (ms * 1e-3) / 2.5e15, 5), "flops": fl}
     net.close()
     return out
 

@@ -339,3 +339,42 @@ def projection_race(R: int, net, seed: int, start_step: int, steps: int, tol: fl
             out[m] = {"sweeps": rec["sweeps"] / n, "seconds": rec["seconds"] / n, "fallbacks": rec["fallbacks"],
                       "projections": rec["projections"], "max_rel": rec["max_rel"]}
     return out
+
+
+def equal_residual_race(R: int, net, seed: int = 7, start_step: int = 100, budget: int = 16384,
+                        warm_sweeps: int = 20000, K: int = 40, check: int = 1024):
+    """The first projection of step start_step + 2 of a seeded 64-vortex flow,
+    after one step whose projections were solved for warm_sweeps sweeps (so
+    the previous pressure is a converged one), from identical states: the
+    relative residual plain Jacobi reaches in `budget` sweeps from zero and
+    from the previous pressure, and the sweeps and time the network's seed
+    needs to reach the previous-pressure start's residual.  Every run checks
+    the residual every `check` sweeps (a host synchronisation each)."""
+    st = W.vortex_state(R, seed=seed)
+
+    def state():
+        s = Solver(R, diffuse_iterations=K, pressure_iterations=K, order=JACOBI)
+        s.upload(*st)
+        s.set_sources(None, None, None)
+        s.step_n(start_step, 1e-5, 1e-6, 0.05)
+        for first in (True, False):
+            if first:
+                s.diffuse(0, 1e-5, 0.05, K); s.advect_density(0.05); s.diffuse(1, 1e-6, 0.05, K)
+                s.diffuse(2, 1e-6, 0.05, K)
+            else:
+                s.advect_velocity(0.05)
+            seeded_projection(s, None, 0 if first else 1, 0.0, warm_sweeps, warm_sweeps)
+        s.diffuse(0, 1e-5, 0.05, K); s.advect_density(0.05); s.diffuse(1, 1e-6, 0.05, K); s.diffuse(2, 1e-6, 0.05, K)
+        return s
+
+    out = {}
+    for name, sd in (("zero", 0), ("previous_pressure", 1)):
+        with state() as s:
+            sw, rel, _, sec = seeded_projection(s, None, sd, 0.0, budget, check)
+        out[name] = {"sweeps": sw, "rel_residual": rel, "ms": sec * 1e3}
+    target = out["previous_pressure"]["rel_residual"]
+    with state() as s:
+        sw, rel, fb, sec = seeded_projection(s, net, 2, target, 4 * budget, check, fallback=False)
+    out["network"] = {"sweeps": sw, "rel_residual": rel, "ms": sec * 1e3, "reached": rel <= target}
+    out["target_rel_residual"] = target
+    return out
