@@ -217,12 +217,16 @@ def projection_inputs(solver: Solver, k_from: int, k_to: int, K: int, diff: floa
         solver.project(K)
 
 
-def make_pairs(R: int, seeds, steps, crop: int, crops_per_pair: int, K: int = 40, rng_seed: int = 0, log=None):
+def make_pairs(R: int, seeds, steps, crop: int, crops_per_pair: int, K: int = 40, solve_sweeps: int = 20000,
+               rng_seed: int = 0, log=None):
     """Training crops from seeded 64-vortex runs of the GPU solver (Jacobi
-    order, K sweeps, dt 0.05, diffusion 1e-5, viscosity 1e-6).  At each step k
-    in `steps` the divergence entering each of its two projections is taken
-    and its pressure solved exactly (poisson_dct); a pair is (divergence, the
-    previous projection's converged pressure, converged pressure), and
+    order, K sweeps, dt 0.05, diffusion 1e-5, viscosity 1e-6).  Between
+    samples the solver takes plain steps; at each sampled step k, steps k-1
+    and k run stage by stage with every projection solved for `solve_sweeps`
+    sweeps (from zero, then from the previous pressure), as a seeded solve to
+    a tight tolerance would leave them.  A pair is (divergence entering a
+    projection of step k, the handle's pressure from the projection before
+    it, the exact pressure of that divergence from poisson_dct), and
     `crops_per_pair` random crop x crop windows of it are kept."""
     rng = np.random.default_rng(rng_seed)
     n = R - 2
@@ -236,19 +240,27 @@ def make_pairs(R: int, seeds, steps, crop: int, crops_per_pair: int, K: int = 40
             for k in steps:
                 if k - 2 > done:
                     s.step_n(k - 2 - done, 1e-5, 1e-6, 0.05)
-                    done = k - 2
-                prev = None
-                for kk, which, d in projection_inputs(s, done, k, K, 1e-5, 1e-6, 0.05):
-                    if kk < k and which == 1:
-                        continue  # only step k-1's second projection seeds a pair
-                    p = poisson_dct(d)
-                    if kk == k and prev is not None:
-                        for _ in range(crops_per_pair):
-                            i, j = rng.integers(0, n - crop + 1, size=2)
-                            D.append(d[i:i + crop, j:j + crop].astype(np.float32))
-                            PV.append(prev[i:i + crop, j:j + crop].astype(np.float32))
-                            T.append(p[i:i + crop, j:j + crop].astype(np.float32))
-                    prev = p
+                for kk in (k - 1, k):
+                    for proj in (1, 2):
+                        if proj == 1:
+                            s.diffuse(0, 1e-5, 0.05, K)
+                            s.advect_density(0.05)
+                            s.diffuse(1, 1e-6, 0.05, K)
+                            s.diffuse(2, 1e-6, 0.05, K)
+                        else:
+                            s.advect_velocity(0.05)
+                        if kk == k:
+                            _, u, v = s.download()
+                            d = divergence(u, v)
+                            prev = s.pressure()[1:-1, 1:-1].astype(np.float64)
+                            p = poisson_dct(d)
+                            for _ in range(crops_per_pair):
+                                i, j = rng.integers(0, n - crop + 1, size=2)
+                                D.append(d[i:i + crop, j:j + crop].astype(np.float32))
+                                PV.append(prev[i:i + crop, j:j + crop].astype(np.float32))
+                                T.append(p[i:i + crop, j:j + crop].astype(np.float32))
+                        seeded_projection(s, None, 0 if (kk == k - 1 and proj == 1) else 1, 0.0, solve_sweeps,
+                                          solve_sweeps)
                 done = k
         if log:
             log(f"seed {seed}: {len(D)} crops")

@@ -3,9 +3,11 @@
     PYTHONPATH=fluid-2d_amd python -m fs2d_amd.train_pressure_net [--out PATH] [--seeds 10] [--epochs 4]
 
 Training data: seeded 64-vortex runs at 1024 x 1024 (the GPU solver, Jacobi
-order, K = 40), both projections of steps 10, 20, ..., 500, labelled by the
-exact DCT solve of the pressure equation; 256 x 256 crops, batch 16, Adam with
-lr 1e-3.  Run on a machine with the GPU; the parameters go to --out (.npz).
+order, K = 40); at steps 10, 20, ..., 500 both projections of steps k-1 and k
+are solved for 20000 sweeps and each projection of step k gives a pair
+(divergence, the handle's previous pressure) labelled by the exact DCT solve
+of the pressure equation; 256 x 256 crops, batch 16, Adam with lr 1e-3.  Run
+on a machine with the GPU; the parameters go to --out (.npz).
 """
 from __future__ import annotations
 
