@@ -133,56 +133,51 @@ int refresh_bf16(fs2d_net* n) {
     return FS2D_OK;
 }
 
-int conv(fs2d_net* n, int l, const bf16* x, void* y, int N, int H, int W, bool f32_out = false) {
-    const Layer& L = n->layer[l];
-    const long npix = static_cast<long>(N) * H * W;
-    const int tiles = L.cout / 16;
+// One 3x3 convolution launch: the LDS-staged kernel where the image tiles into
+// 8 x 16 pixel blocks, the direct one otherwise (the small 1/8-scale levels).
+int launch_conv(fs2d_net* n, const bf16* x, int N, int H, int W, int Cin, const bf16* w, const float* bias, void* y,
+                int Cout, int relu, bool f32_out) {
+    const int tiles = Cout / 16;
     const int nt = tiles % 4 == 0 ? 4 : (tiles % 3 == 0 ? 3 : (tiles % 2 == 0 ? 2 : 1));
-    const dim3 grid(static_cast<unsigned>((npix / 16 + CONV_WAVES - 1) / CONV_WAVES), tiles / nt);
-    const dim3 blk(64 * CONV_WAVES);
-    const int relu = L.relu ? 1 : 0;
-    if (f32_out)
-        hipLaunchKernelGGL((k_conv3x3<1, true>), grid, blk, 0, n->stream, x, N, H, W, L.cin, L.w16, L.b(), y, L.cout,
-                           relu);
-    else if (nt == 4)
-        hipLaunchKernelGGL((k_conv3x3<4, false>), grid, blk, 0, n->stream, x, N, H, W, L.cin, L.w16, L.b(), y, L.cout,
-                           relu);
-    else if (nt == 3)
-        hipLaunchKernelGGL((k_conv3x3<3, false>), grid, blk, 0, n->stream, x, N, H, W, L.cin, L.w16, L.b(), y, L.cout,
-                           relu);
-    else if (nt == 2)
-        hipLaunchKernelGGL((k_conv3x3<2, false>), grid, blk, 0, n->stream, x, N, H, W, L.cin, L.w16, L.b(), y, L.cout,
-                           relu);
-    else
-        hipLaunchKernelGGL((k_conv3x3<1, false>), grid, blk, 0, n->stream, x, N, H, W, L.cin, L.w16, L.b(), y, L.cout,
-                           relu);
+    const dim3 blk(256);
+    if (H % 8 == 0 && W % 16 == 0) {
+        const dim3 grid(static_cast<unsigned>(static_cast<long>(N) * (H / 8) * (W / 16)), tiles / nt);
+#define FS2D_CONV_LDS(NT_, F32_) \
+    hipLaunchKernelGGL((k_conv3x3_lds<NT_, F32_>), grid, blk, 0, n->stream, x, N, H, W, Cin, w, bias, y, Cout, relu)
+        if (f32_out && nt == 1) FS2D_CONV_LDS(1, true);
+        else if (f32_out) return nfail(n, FS2D_ERR_ARGUMENT, "conv: fp32 output needs 16 channels");
+        else if (nt == 4) FS2D_CONV_LDS(4, false);
+        else if (nt == 3) FS2D_CONV_LDS(3, false);
+        else if (nt == 2) FS2D_CONV_LDS(2, false);
+        else FS2D_CONV_LDS(1, false);
+#undef FS2D_CONV_LDS
+    } else {
+        const long npix = static_cast<long>(N) * H * W;
+        const dim3 grid(static_cast<unsigned>((npix / 16 + CONV_WAVES - 1) / CONV_WAVES), tiles / nt);
+#define FS2D_CONV(NT_, F32_) \
+    hipLaunchKernelGGL((k_conv3x3<NT_, F32_>), grid, blk, 0, n->stream, x, N, H, W, Cin, w, bias, y, Cout, relu)
+        if (f32_out && nt == 1) FS2D_CONV(1, true);
+        else if (f32_out) return nfail(n, FS2D_ERR_ARGUMENT, "conv: fp32 output needs 16 channels");
+        else if (nt == 4) FS2D_CONV(4, false);
+        else if (nt == 3) FS2D_CONV(3, false);
+        else if (nt == 2) FS2D_CONV(2, false);
+        else FS2D_CONV(1, false);
+#undef FS2D_CONV
+    }
     NHIP(n, hipGetLastError());
     return FS2D_OK;
+}
+
+int conv(fs2d_net* n, int l, const bf16* x, void* y, int N, int H, int W, bool f32_out = false) {
+    const Layer& L = n->layer[l];
+    return launch_conv(n, x, N, H, W, L.cin, L.w16, L.b(), y, L.cout, L.relu ? 1 : 0, f32_out);
 }
 
 // Data gradient of layer l: dx = conv(dz, flipped transposed weights).
 int conv_dx(fs2d_net* n, int l, const bf16* dz, bf16* dx, int N, int H, int W) {
     const Layer& L = n->layer[l];
     hipLaunchKernelGGL(k_flip_weights, dim3(blocks_for(L.wn)), dim3(256), 0, n->stream, L.w16, L.wt16, L.cin, L.cout);
-    const long npix = static_cast<long>(N) * H * W;
-    const int tiles = L.cin / 16;
-    const int nt = tiles % 4 == 0 ? 4 : (tiles % 3 == 0 ? 3 : (tiles % 2 == 0 ? 2 : 1));
-    const dim3 grid(static_cast<unsigned>((npix / 16 + CONV_WAVES - 1) / CONV_WAVES), tiles / nt);
-    const dim3 blk(64 * CONV_WAVES);
-    if (nt == 4)
-        hipLaunchKernelGGL((k_conv3x3<4, false>), grid, blk, 0, n->stream, dz, N, H, W, L.cout, L.wt16, nullptr, dx,
-                           L.cin, 0);
-    else if (nt == 3)
-        hipLaunchKernelGGL((k_conv3x3<3, false>), grid, blk, 0, n->stream, dz, N, H, W, L.cout, L.wt16, nullptr, dx,
-                           L.cin, 0);
-    else if (nt == 2)
-        hipLaunchKernelGGL((k_conv3x3<2, false>), grid, blk, 0, n->stream, dz, N, H, W, L.cout, L.wt16, nullptr, dx,
-                           L.cin, 0);
-    else
-        hipLaunchKernelGGL((k_conv3x3<1, false>), grid, blk, 0, n->stream, dz, N, H, W, L.cout, L.wt16, nullptr, dx,
-                           L.cin, 0);
-    NHIP(n, hipGetLastError());
-    return FS2D_OK;
+    return launch_conv(n, dz, N, H, W, L.cout, L.wt16, nullptr, dx, L.cin, 0, false);
 }
 
 // Weight and bias gradients of layer l (accumulated into its gradient arena).

@@ -5,8 +5,8 @@
 // stride 1) is an implicit GEMM D[p][co] = sum_{tap, ci} X[p + tap][ci] *
 // Wt[tap][co][ci] on v_mfma_f32_16x16x16bf16_1k: lane l supplies
 // A[m = l % 16][k = 4 (l / 16) + i] and B[k = 4 (l / 16) + i][n = l % 16] and
-// holds D[m = 4 (l / 16) + i][n = l % 16], i = 0..3 (checked on the device by
-// tests/test_gpu_cnn.py against a host matrix product).
+// holds D[m = 4 (l / 16) + i][n = l % 16], i = 0..3 (the network built from
+// these kernels is checked on the device against a PyTorch reference).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -89,6 +89,89 @@ __global__ __launch_bounds__(64 * CONV_WAVES) void k_conv3x3(const bf16* __restr
                 static_cast<float*>(y)[p * Cout + co] = v;
             else
                 static_cast<bf16*>(y)[p * Cout + co] = f2bf(v);
+        }
+    }
+}
+
+// The same convolution with the operands staged in LDS.  A block computes an
+// 8-row x 16-column pixel tile (W a multiple of 16, H of 8) for 16 NT output
+// channels; wave w takes rows 2w and 2w + 1, so each B fragment feeds two
+// MFMAs.  Per 16-channel input chunk the block stages the tile's 10 x 18 input
+// halo (zero outside the image) and the chunk's 9 x 16 NT x 16 weights in LDS.
+template <int NT, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(const bf16* __restrict__ x, int N, int H, int W, int Cin,
+                                                    const bf16* __restrict__ w, const float* __restrict__ bias,
+                                                    void* __restrict__ y, int Cout, int relu) {
+    constexpr int TR = 8, TC = 16, HR = TR + 2, HC = TC + 2;
+    constexpr int CO = 16 * NT;
+    __shared__ short4_t xs[HR * HC * 4];  // [row][col][16 channels as 4 x short4]
+    __shared__ short4_t ws[9 * CO * 4];   // [tap][co][16 channels as 4 x short4]
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int tiles_x = W / TC, tiles_y = H / TR;
+    int b = blockIdx.x;
+    const int tx = b % tiles_x;
+    b /= tiles_x;
+    const int ty = b % tiles_y;
+    const long n = b / tiles_y;
+    const int x0 = tx * TC, y0 = ty * TR;
+    const int co0 = blockIdx.y * CO;
+    const int kq4 = l >> 4;  // which short4 of the 16-channel chunk this lane supplies
+    float4_t acc[2][NT];
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[g][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < Cin; c0 += 16) {
+        __syncthreads();  // the previous chunk's reads are done
+        for (int q = tid; q < HR * HC * 4; q += 256) {
+            const int part = q & 3, pix = q >> 2;
+            const int r = pix / HC, c = pix % HC;
+            const int yy = y0 + r - 1, xx = x0 + c - 1;
+            short4_t v = short4_t{0, 0, 0, 0};
+            if ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W)
+                v = *reinterpret_cast<const short4_t*>(x + ((n * H + yy) * W + xx) * Cin + c0 + 4 * part);
+            xs[q] = v;
+        }
+        for (int q = tid; q < 9 * CO * 4; q += 256) {
+            const int part = q & 3, tc = q >> 2;
+            const int co = tc % CO, tap = tc / CO;
+            ws[q] = *reinterpret_cast<const short4_t*>(w + (static_cast<long>(tap) * Cout + co0 + co) * Cin + c0 +
+                                                       4 * part);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;  // offsets into the halo (0..2)
+            short4_t a[2];
+#pragma unroll
+            for (int g = 0; g < 2; ++g) a[g] = xs[((2 * wave + g + dy) * HC + (l & 15) + dx) * 4 + kq4];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const short4_t bb = ws[(tap * CO + 16 * t + (l & 15)) * 4 + kq4];
+#pragma unroll
+                for (int g = 0; g < 2; ++g)
+                    acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[g], bb, acc[g][t], 0, 0, 0);
+            }
+        }
+    }
+    // D[m = 4 (l / 16) + i][n = l % 16]: pixel column x0 + m of row y0 + 2 wave + g
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const long prow = (n * H + y0 + 2 * wave + g) * W + x0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int co = co0 + 16 * t + (l & 15);
+            const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[g][t][i] + bv;
+                if (relu) v = fmaxf(v, 0.f);
+                const long p = prow + 4 * (l >> 4) + i;
+                if (OUT_F32)
+                    static_cast<float*>(y)[p * Cout + co] = v;
+                else
+                    static_cast<bf16*>(y)[p * Cout + co] = f2bf(v);
+            }
         }
     }
 }

@@ -107,7 +107,7 @@ def test_gradients_match_torch(sample):
         loss = net.train_step(d, prev, target, lr=0.0)
         g = net.grads()
     _, _, tloss, layers = torch_net(p, d, prev, target)
-    assert abs(loss - float(tloss)) <= 2e-2 * float(tloss)
+    assert abs(loss - float(tloss.detach())) <= 2e-2 * float(tloss.detach())
     for k, ((gw, gb), (wt, bt)) in enumerate(zip(PN.split_params(g), layers)):
         tw = wt.grad.permute(2, 3, 0, 1).reshape(9, gw.shape[1], gw.shape[2]).numpy()
         tb = bt.grad.numpy()

@@ -358,35 +358,38 @@ def big_state():
     return W.vortex_state(R, seed=3)
 
 
-@pytest.mark.parametrize("gsb,lag", [(1, 0), (4, 1), (8, 8)])
-def test_full_size_stages(big_state, gsb, lag, oracle_lib):
+@pytest.mark.parametrize("order,gsb,lag,block,K", [(GS, 1, 0, 0, 3), (GS, 4, 1, 0, 3), (GS, 8, 8, 0, 3),
+                                                   (JAC, 0, 0, 4, 8), (JAC, 0, 0, 8, 16)])
+def test_full_size_stages(big_state, order, gsb, lag, block, K, oracle_lib):
     """4096^2 interior: each stage against the oracle with short solves (the
     oracle cannot afford 80 sweeps here); exercises the full tile grid."""
     R = 4098
     rho, u, v = big_state
     z = np.zeros((R, R), np.float32)
     st = (rho, u, v, z, z, z)
-    with make(R, st, kd=3, kp=3, gsb=gsb, lag=lag) as s:
-        s.diffuse(0, 1e-5, 0.05, 3)
+    with make(R, st, order=order, kd=K, kp=K, gsb=gsb, lag=lag, block=block) as s:
+        s.diffuse(0, 1e-5, 0.05, K)
         s.advect_density(0.05)
-        s.project(3)
+        s.project(K)
         got = s.download()
     want = [rho.copy(), u.copy(), v.copy()]
-    oracle_lib.diffuse(want[0], 0, 1e-5, 0.05, 3)
+    oracle_lib.diffuse(want[0], 0, 1e-5, 0.05, K, oracle_order(order))
     oracle_lib.advect(want[0], want[1], want[2], 0, 0.05, True)
-    oracle_lib.project(want[1], want[2], 3)
+    oracle_lib.project(want[1], want[2], K, oracle_order(order))
     for f in range(3):
-        assert_same(got[f], want[f], f"4096^2 stages field {f}")
+        assert_same(got[f], want[f], f"4096^2 stages field {f} order={order} block={block}")
 
 
-def test_full_size_step_deterministic(big_state):
-    """Two independent runs of 3 full steps (K = 80) are bit-identical, and the
-    second projection leaves the wide-stencil divergence below its input."""
+@pytest.mark.parametrize("order", [GS, JAC])
+def test_full_size_step_deterministic(big_state, order):
+    """Two independent runs of 3 full steps (K = 80) are bit-identical (in the
+    Jacobi order whatever depths the autotuner picks in each), and the second
+    projection leaves the wide-stencil divergence below its input."""
     R = 4098
     rho, u, v = big_state
     out = []
     for _ in range(2):
-        with fs2d_amd.Solver(R, diffuse_iterations=80, pressure_iterations=80) as s:
+        with fs2d_amd.Solver(R, diffuse_iterations=80, pressure_iterations=80, order=order) as s:
             s.upload(rho, u, v)
             s.set_sources(None, None, None)
             s.step_n(3, 1e-5, 1e-6, 0.05)
