@@ -402,9 +402,15 @@ __global__ __launch_bounds__(256) void k_sumsq2(const float* __restrict__ d, con
         a += __shfl_xor(a, off);
         b += __shfl_xor(b, off);
     }
+    __shared__ double part[2][4];
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(scal, a);
-        atomicAdd(scal + 1, b);
+        part[0][threadIdx.x >> 6] = a;
+        part[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one pair of atomics per block
+        atomicAdd(scal, (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]));
+        atomicAdd(scal + 1, (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]));
     }
 }
 
@@ -471,7 +477,8 @@ int net_predict_fields(fs2d_net* net, hipStream_t stream, const float* d, const 
     hipStream_t saved = net->stream;
     net->stream = stream;
     const long cells = static_cast<long>(n) * n;
-    hipLaunchKernelGGL(k_sumsq2, dim3(blocks_for(cells)), dim3(256), 0, stream, d, prev, R, P, net->scal);
+    hipLaunchKernelGGL(k_sumsq2, dim3(std::min(blocks_for(cells), 1024u)), dim3(256), 0, stream, d, prev, R, P,
+                       net->scal);
     hipLaunchKernelGGL(k_scale, dim3(1), dim3(1), 0, stream, net->scal, cells, a->f("scale"));
     hipLaunchKernelGGL(k_pack_fields, dim3(blocks_for(cells)), dim3(256), 0, stream, d, prev, R, P, a->f("scale"),
                        (*a)("x0"), a->f("prev"));

@@ -143,9 +143,15 @@ __global__ __launch_bounds__(256) void k_pressure_residual(const float* __restri
         rr += __shfl_xor(rr, off);
         dd += __shfl_xor(dd, off);
     }
+    __shared__ double part[2][4];
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(sums, rr);
-        atomicAdd(sums + 1, dd);
+        part[0][threadIdx.x >> 6] = rr;
+        part[1][threadIdx.x >> 6] = dd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one pair of atomics per block
+        atomicAdd(sums, (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]));
+        atomicAdd(sums + 1, (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]));
     }
 }
 
@@ -208,7 +214,7 @@ hipError_t launch_set_boundary(hipStream_t s, float* x, int R, int P, Ring ring)
 
 hipError_t launch_pressure_residual(hipStream_t s, const float* d, const float* p, int R, int P, double* sums) {
     const long cells = static_cast<long>(R - 2) * (R - 2);
-    const unsigned blocks = static_cast<unsigned>(std::max<long>(1, std::min<long>((cells + 255) / 256, 2048)));
+    const unsigned blocks = static_cast<unsigned>(std::max<long>(1, std::min<long>((cells + 255) / 256, 1024)));
     hipLaunchKernelGGL(k_pressure_residual, dim3(blocks), dim3(256), 0, s, d, p, R, P, sums);
     return hipGetLastError();
 }
