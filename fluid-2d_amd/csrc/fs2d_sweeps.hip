@@ -26,6 +26,8 @@
 //    per launch (temporal blocking): a 256-thread block loads its tile plus an
 //    M-cell halo into LDS once, relaxes M times in LDS on a shrinking region
 //    (recomputing the ring in LDS after each sweep), and writes only the tile.
+#include <type_traits>
+
 #include "fs2d_internal.hpp"
 #include "fs2d_jacobi.hpp"
 
@@ -239,11 +241,10 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     constexpr int BP = BW + (BW & 1);
     constexpr int NX = (XH * XW + NT - 1) / NT, NBX = (BH * BW + NT - 1) / NT;
     // Guard space after each array absorbs the operand reads of inactive lanes
-    // (their skewed columns run up to GS_ROWS + TW + M past the cell); the
-    // DUMMY slots, one per thread, take the stores of lanes that store nothing.
+    // (their skewed columns run up to GS_ROWS + TW + M past the cell); lanes
+    // that relax nothing in a step store nothing.
     constexpr int GUARD = GS_ROWS + TW + 2 * M + 8;
-    constexpr int DUMMY0 = XH * XP + GUARD;
-    __shared__ float X[XH * XP + GUARD + NT];
+    __shared__ float X[XH * XP + GUARD];
     __shared__ float Bs[BH * BP + GUARD];
 
     const int2 wk = work[blockIdx.x - f * S.units_per_field];
@@ -285,7 +286,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     }
 
     // ---- the M sweeps
-    const int w = tid >> 6, r = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), r = tid & 63;
     const int k = k_lo + w;
     const int i = GS_ROWS * I + 1 + r - k;
     const bool wave_ok = (k <= k_hi);
@@ -309,43 +310,66 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     // with only the DPP / register chain between steps.
     constexpr int STEPS = GS_ROWS + TW - 1;
     constexpr int NINT = (STEPS + L * (M - 1) + L - 1) / L;
-    // The tick is branch-free: a lane that relaxes nothing this tick, and a
-    // lane whose cell borders no ring, stores to its own dummy LDS slot, so
-    // the batched operand reads cannot be sunk into per-tick branches.
+    // The operand reads of an interval are pinned ahead of its relaxations
+    // (sched_barrier), so they are never sunk into per-tick branches.
     const int me0 = (1 + r + (M - 1 - w)) * XP + ofs - r;  // X index of this lane's cell at step s: me0 + s
     const int bme0 = (r + (M - 1 - w)) * BP + ofs - 1 - r;  // Bs index of its right-hand side: bme0 + s
-    const int dummy = DUMMY0 + tid;
     if (plain) {
+        // Step phases: while s < 64, lane r's first cell (c = 0, whose west
+        // neighbour is the tile halo) is at step r and lanes r <= s are
+        // active; from s = 64 on every lane has passed column 0 and lanes
+        // r >= s - 63 are active.  Intervals start at multiples of L, so an
+        // interval lies in one phase.  The north halo matters only to lane
+        // 0, which is active only while s < 64; it enters as the DPP shift's
+        // old value (lane 0 has no source lane).
+        static_assert(TW == 64 && GS_ROWS == 64 && 64 % L == 0, "the step phases assume 64 x 64 tiles");
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
             // a wave with no step of this interval in [0, STEPS) only joins the barrier
             if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
-            float so[L], e[L], bb[L], nh[L], wh[L];
+            const int x0 = me0 + s0, b0 = bme0 + s0;
+            float so[L], e[L], bb[L];
+            if (s0 < TW) {
+                float nh[L], wh[L];
 #pragma unroll
-            for (int q = 0; q < L; ++q) {
-                const int x = me0 + s0 + q;
-                so[q] = X[x + XP];
-                e[q] = X[x + 1];
-                bb[q] = Bs[bme0 + s0 + q];
-                nh[q] = X[x - XP];
-                wh[q] = X[x - 1];
-            }
-            __builtin_amdgcn_sched_barrier(0);
+                for (int q = 0; q < L; ++q) {
+                    so[q] = X[x0 + q + XP];
+                    e[q] = X[x0 + q + 1];
+                    bb[q] = Bs[b0 + q];
+                    nh[q] = X[x0 + q - XP];
+                    wh[q] = X[x0 + q - 1];
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int q = 0; q < L; ++q) {
-                const int s = s0 + q;
-                const int c = s - r;
-                const bool active = (unsigned)c < (unsigned)TW;
-                const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                const float nv = (r == 0) ? nh[q] : from_up;
-                const float wv = (c == 0) ? wh[q] : xprev;
-                const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, dv);
-                X[active ? me0 + s : dummy] = v;
-                // xprev is consumed only where this lane was active one tick
-                // earlier (as W) or lane r + 1 is active now (as N, same
-                // column), so it need not be held across inactive ticks
-                xprev = v;
+                for (int q = 0; q < L; ++q) {
+                    const int s = s0 + q;
+                    const float nv = __int_as_float(
+                        __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                    const float wv = (r == s) ? wh[q] : xprev;
+                    const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, dv);
+                    if (r <= s) X[x0 + q] = v;
+                    // xprev is consumed only where this lane was active one
+                    // tick earlier (as W) or lane r + 1 is active now (as N,
+                    // same column), so it need not be held across inactive ticks
+                    xprev = v;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < L; ++q) {
+                    so[q] = X[x0 + q + XP];
+                    e[q] = X[x0 + q + 1];
+                    bb[q] = Bs[b0 + q];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < L; ++q) {
+                    const int s = s0 + q;
+                    const float nv = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                    const float v = relax_k<KIND>(bb[q], nv, so[q], xprev, e[q], a, dv);
+                    if (r >= s - (TW - 1)) X[x0 + q] = v;
+                    xprev = v;
+                }
             }
         }
     } else {
@@ -354,53 +378,78 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
         // (read with the other operands; the cell is overwritten only after
         // its own relaxation), and for sweep 0 the stored ring, which the
         // unit loaded.  Write-back stores the ring of every edge cell, so the
-        // next unit finds the ring of the latest sweep there.
+        // next unit finds the ring of the latest sweep there.  The loop is
+        // specialised by the edges the unit meets (top/bottom rows, left/right
+        // columns); the tests it leaves out cannot hold in such a unit.
         const bool row_ok = wave_ok && i >= 1 && i <= n;
-        const bool top = (i == 1), bot = (i == n), kpos = (k > 0);
+        const bool kpos = (k > 0);
+        const bool top = row_ok && i == 1, bot = row_ok && i == n;
         const int c_left = 1 - jbase, c_right = n - jbase;  // skewed columns of j = 1 and j = n
-        const int c_lo = max(0, c_left), c_hi = min(TW - 1, c_right);
+        const int c_lo = row_ok ? max(0, c_left) : TW, c_hi = min(TW - 1, c_right);
         const unsigned tb_sign = rg.neg_tb ? 0x80000000u : 0u, lr_sign = rg.neg_lr ? 0x80000000u : 0u;
-        const bool n_ring = top && kpos;                   // N is the virtual ring
-        const bool n_lds_lane = (r == 0) || top;           // else from LDS: tile halo, or stored ring (k = 0)
-        for (int it = 0; it < NINT; ++it) {
-            __syncthreads();
-            const int s0 = it * L - L * w;
-            if (!row_ok || s0 + L <= 0 || s0 >= STEPS) continue;
-            float s_lds[L], e_lds[L], bb[L], n_lds[L], w_lds[L], self[L];
+        const bool n_virtual = top && kpos, s_virtual = bot && kpos;
+        // W at the lane's first column: the tile halo, or on the left edge
+        // the ring (virtual for k > 0); wave-uniform
+        const bool w_virtual = kpos && c_left >= 0;
+        const bool rows_edge = GS_ROWS * I + 1 - k_hi <= 1 || GS_ROWS * I + GS_ROWS - k_lo >= n;
+        const bool cols_edge = TW * J + 1 - k_hi <= 1 || TW * J + TW - k_lo >= n;
+        auto sweeps = [&](auto rows_t, auto cols_t) {
+            constexpr bool RE = decltype(rows_t)::value, CE = decltype(cols_t)::value;
+            for (int it = 0; it < NINT; ++it) {
+                __syncthreads();
+                const int s0 = it * L - L * w;
+                if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
+                const int x0 = me0 + s0, b0 = bme0 + s0;
+                float so[L], e[L], bb[L], nh[L], wh[L], self[L];
 #pragma unroll
-            for (int q = 0; q < L; ++q) {
-                const int x = me0 + s0 + q;
-                s_lds[q] = X[x + XP];
-                e_lds[q] = X[x + 1];
-                bb[q] = Bs[bme0 + s0 + q];
-                n_lds[q] = X[x - XP];
-                w_lds[q] = X[x - 1];
-                self[q] = X[x];
-            }
-            __builtin_amdgcn_sched_barrier(0);
+                for (int q = 0; q < L; ++q) {
+                    so[q] = X[x0 + q + XP];
+                    e[q] = X[x0 + q + 1];
+                    bb[q] = Bs[b0 + q];
+                    nh[q] = X[x0 + q - XP];
+                    wh[q] = X[x0 + q - 1];
+                    self[q] = X[x0 + q];
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int q = 0; q < L; ++q) {
-                const int s = s0 + q;
-                const int c = s - r;
-                const bool active = c >= c_lo && c <= c_hi;
-                const float from_up = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                const float tbv = __uint_as_float(__float_as_uint(self[q]) ^ tb_sign);
-                const float lrv = __uint_as_float(__float_as_uint(self[q]) ^ lr_sign);
-                const float nv = n_ring ? tbv : (n_lds_lane ? n_lds[q] : from_up);
-                const float sv = (bot && kpos) ? tbv : s_lds[q];
-                const float wv = (c == c_left && kpos) ? lrv : ((c == 0 || c == c_left) ? w_lds[q] : xprev);
-                const float ev = (c == c_right && kpos) ? lrv : e_lds[q];
-                const float v = relax_k<KIND>(bb[q], nv, sv, wv, ev, a, dv);
-                X[active ? me0 + s : dummy] = v;
-                xprev = v;
+                for (int q = 0; q < L; ++q) {
+                    const int c = s0 + q - r;
+                    // lane 0 receives the tile halo as the shift's old value
+                    const float up = __int_as_float(
+                        __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                    float nv = up, sv = so[q], ev = e[q], wv;
+                    if constexpr (RE) {
+                        const float tbv = __uint_as_float(__float_as_uint(self[q]) ^ tb_sign);
+                        nv = n_virtual ? tbv : (top ? nh[q] : up);
+                        sv = s_virtual ? tbv : so[q];
+                    }
+                    if constexpr (CE) {
+                        const float lrv = __uint_as_float(__float_as_uint(self[q]) ^ lr_sign);
+                        wv = (c == c_lo) ? (w_virtual ? lrv : wh[q]) : xprev;
+                        ev = (kpos && c == c_right) ? lrv : e[q];
+                    } else {
+                        wv = (c == c_lo) ? wh[q] : xprev;
+                    }
+                    const float v = relax_k<KIND>(bb[q], nv, sv, wv, ev, a, dv);
+                    if (c >= c_lo && c <= c_hi) X[x0 + q] = v;
+                    xprev = v;
+                }
             }
-        }
+        };
+        using yes = std::integral_constant<bool, true>;
+        using no = std::integral_constant<bool, false>;
+        if (rows_edge && cols_edge)
+            sweeps(yes{}, yes{});
+        else if (rows_edge)
+            sweeps(yes{}, no{});
+        else
+            sweeps(no{}, yes{});
     }
     __syncthreads();
 
-    // ---- write back every cell this unit relaxed (its latest sweep); edge
-    // cells also write the ring they border (after the solve's last sweep
-    // that is the final ring).
+    // ---- write back every cell this unit relaxed (its latest sweep).  Edge
+    // cells of a unit that meets the ring also write the ring they border
+    // (after the solve's last sweep that is the final ring).
     for (int idx = tid; idx < XH * XW; idx += NT) {
         const int li = idx / XW, lj = idx % XW;
         const int gi = gi_lo + li, gj = gj_lo + lj;
@@ -410,7 +459,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
         const float v = X[li * XP + lj];
         const size_t o = (size_t)gi * P + gj;
         x[o] = v;
-        {  // the ring of the cell's latest sweep: what the next unit reading it expects
+        if (!plain) {  // the ring of the cell's latest sweep: what the next unit reading it expects
             const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
             const float cn = 0.5f * (tb + lr);
             if (gi == 1) x[o - P] = tb;

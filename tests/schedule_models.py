@@ -489,19 +489,25 @@ def gs_unit_lag_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, L,
     return x
 
 
-def gs_unit_ring_model(x, b, K, kind, a, denom, neg_tb, neg_lr, M, L, TW=64):
+def gs_unit_ring_model(x, b, K, kind, a, denom, neg_tb, neg_lr, M, L, TW=64, lag=None, prefetch=False):
     """k_gs_unit, final form: lagged waves (as gs_unit_lag_model) with the ring
     kept as data instead of logic.  Each relaxed edge cell also writes its
     sign-adjusted ring cell -- in LDS for the next sweep of the same unit, and
     to the grid at write-back for the next unit -- so every neighbour is read
     the same way.  A zero-start solve (pressure) is run on an iterate that the
-    caller zero-filled, ring included."""
+    caller zero-filled, ring included.
+
+    lag: ticks by which wave w + 1 trails wave w (default L).  prefetch: each
+    wave reads the operands of an interval right after the barrier that opens
+    the interval before it (so the reads overlap that interval's relaxations);
+    asserted race-free over both intervals."""
     R = x.shape[0]
     n = R - 2
     a, denom = F(a), F(denom)
     XH, XW = ROWS + 1 + M, TW + 1 + M
     STEPS = ROWS + TW - 1
-    TICKS = STEPS + L * (M - 1)
+    D = L if lag is None else lag
+    TICKS = STEPS + D * (M - 1)
     for level in gs_unit_plan(R, K, TW, M):
         snap = x.copy()
         reads, writes, updates = [], [], []
@@ -514,12 +520,13 @@ def gs_unit_ring_model(x, b, K, kind, a, denom, neg_tb, neg_lr, M, L, TW=64):
                 for lj in range(XW):
                     X[li, lj] = snap[min(max(gi_lo + li, 0), R - 1), min(max(gj_lo + lj, 0), R - 1)]
             xprev = np.zeros((M, ROWS), np.float32)
-            for b0 in range(0, TICKS, L):
+
+            def capture(b0):
                 cap, cap_cells = {}, {w: set() for w in range(M)}
                 for w in range(M):
                     k = kap * M + w
                     for q in range(L):
-                        s = b0 + q - L * w
+                        s = b0 + q - D * w
                         for r in range(ROWS):
                             c = s - r
                             i, j = ROWS * I + 1 + r - k, TW * J + 1 + c - k
@@ -534,11 +541,21 @@ def gs_unit_ring_model(x, b, K, kind, a, denom, neg_tb, neg_lr, M, L, TW=64):
                                 cap_cells[w].add((pi, pj))
                                 rd.add((gi_lo + pi, gj_lo + pj))
                             cap[(w, q, r)] = ops
+                return cap, cap_cells
+
+            pending = capture(0) if prefetch else None
+            wrote_before = {w: set() for w in range(M)}
+            for b0 in range(0, TICKS, L):
+                if prefetch:
+                    cap, cap_cells = pending
+                    pending = capture(b0 + L)
+                else:
+                    cap, cap_cells = capture(b0)
                 wrote = {w: set() for w in range(M)}
                 for q in range(L):
                     for w in range(M):
                         k = kap * M + w
-                        s = b0 + q - L * w
+                        s = b0 + q - D * w
                         from_up = np.concatenate([[F(0)], xprev[w, :-1]])
                         new_prev = xprev[w].copy()
                         for r in range(ROWS):
@@ -561,10 +578,14 @@ def gs_unit_ring_model(x, b, K, kind, a, denom, neg_tb, neg_lr, M, L, TW=64):
                                     wrote[w].add((pi, pj))
                             new_prev[r] = v
                         xprev[w] = new_prev
+                # no other wave writes an operand between its read and its use
                 for w in range(M):
                     for w2 in range(M):
                         if w != w2:
                             assert not (cap_cells[w] & wrote[w2]), ("interval race", b0, w, w2)
+                            if prefetch:
+                                assert not (cap_cells[w] & wrote_before[w2]), ("prefetch race", b0, w, w2)
+                wrote_before = wrote
             out = {}
             for li in range(XH):
                 for lj in range(XW):

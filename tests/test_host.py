@@ -154,7 +154,7 @@ def test_gs_unit_lagged_schedule_reproduces_gauss_seidel(R, M, L, oracle_lib):
     assert bits_equal(p, p_ref)
 
 
-@pytest.mark.parametrize("R,M,L", [(3, 2, 1), (10, 4, 4), (67, 2, 8), (70, 8, 8), (70, 4, 4)])
+@pytest.mark.parametrize("R,M,L", [(3, 2, 1), (10, 4, 4), (67, 2, 8), (70, 8, 8), (70, 4, 4), (70, 8, 4), (131, 8, 4)])
 def test_gs_unit_ring_as_data_reproduces_gauss_seidel(R, M, L, oracle_lib):
     """The kernel's final form: lagged waves, ring cells written as data by
     the edge cells (in LDS and at write-back), zero-filled pressure start."""
@@ -173,6 +173,44 @@ def test_gs_unit_ring_as_data_reproduces_gauss_seidel(R, M, L, oracle_lib):
     p = np.zeros((R, R), np.float32)
     gs_unit_ring_model(p, d, 9, 1, 0, 0, False, False, M, L)
     assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M,L", [(3, 2, 1), (10, 4, 2), (67, 2, 4), (70, 8, 4), (70, 4, 2), (131, 8, 2)])
+def test_gs_unit_prefetch_schedule_reproduces_gauss_seidel(R, M, L, oracle_lib):
+    """The prefetching unit: wave w + 1 trails wave w by 2L ticks, and each
+    wave reads an interval's operands one interval ahead.  Bit-identical to
+    GS-lex, and no other wave writes an operand between its read and use."""
+    rho, u, v, *_ = W.random_state(R, seed=R + 5)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        for K in (1, 9):
+            x = rho.copy()
+            gs_unit_ring_model(x, rho.copy(), K, 0, a, den, mode == 2, mode == 1, M, L, lag=2 * L, prefetch=True)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert bits_equal(x, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
+    p = np.zeros((R, R), np.float32)
+    gs_unit_ring_model(p, d, 9, 1, 0, 0, False, False, M, L, lag=2 * L, prefetch=True)
+    assert bits_equal(p, p_ref)
+
+
+def test_gs_unit_prefetch_needs_the_longer_lag(oracle_lib):
+    """With a lag of L ticks, an operand read one interval ahead has not been
+    written yet: the model's race check (or the bit comparison) catches it."""
+    R, M, L = 70, 4, 4
+    rho, *_ = W.random_state(R, seed=9)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    x = rho.copy()
+    try:
+        gs_unit_ring_model(x, rho.copy(), 9, 0, a, den, False, False, M, L, lag=L, prefetch=True)
+    except AssertionError:
+        return
+    ref = rho.copy(); oracle_lib.diffuse(ref, 0, 5e-2, 0.1, 9, oracle.GS_LEX)
+    assert not bits_equal(x, ref)
 
 
 @pytest.mark.parametrize("R", [3, 10, 67])
