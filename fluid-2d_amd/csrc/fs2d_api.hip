@@ -437,8 +437,8 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: launch_mode must be 0 (graph) or 1 (eager)");
     if (prm.gs_lag != 0 && prm.gs_lag != 1 && prm.gs_lag != 4 && prm.gs_lag != 8)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_lag must be 0, 1, 4 or 8");
-    if (prm.gs_block > 1 && prm.gs_tile_cols == 128)
-        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_block > 1 needs gs_tile_cols 64");
+    if (prm.gs_block == 2 && prm.gs_tile_cols == 128)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_tile_cols 128 takes gs_block 1, 4 or 8");
     if (prm.jacobi_kernel != 0 && prm.jacobi_kernel != FS2D_JACOBI_WAVE && prm.jacobi_kernel != FS2D_JACOBI_LDS)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: jacobi_kernel must be 0, 1 (wave) or 2 (LDS)");
 

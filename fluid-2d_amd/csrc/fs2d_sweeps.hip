@@ -315,14 +315,15 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     const int me0 = (1 + r + (M - 1 - w)) * XP + ofs - r;  // X index of this lane's cell at step s: me0 + s
     const int bme0 = (r + (M - 1 - w)) * BP + ofs - 1 - r;  // Bs index of its right-hand side: bme0 + s
     if (plain) {
-        // Step phases: while s < 64, lane r's first cell (c = 0, whose west
-        // neighbour is the tile halo) is at step r and lanes r <= s are
-        // active; from s = 64 on every lane has passed column 0 and lanes
-        // r >= s - 63 are active.  Intervals start at multiples of L, so an
-        // interval lies in one phase.  The north halo matters only to lane
-        // 0, which is active only while s < 64; it enters as the DPP shift's
+        // Step phases (lane r relaxes column c = s - r): while s < 64, lane
+        // r's first cell (c = 0, whose west neighbour is the tile halo) is at
+        // step r and lanes r <= s are active; while 64 <= s < TW every lane
+        // is inside the tile (c in [1, TW - 1]); from s = TW on lanes
+        // r >= s - (TW - 1) are active.  Intervals start at multiples of L,
+        // so an interval lies in one phase.  The north halo matters only to
+        // lane 0, which is active while s < TW; it enters as the DPP shift's
         // old value (lane 0 has no source lane).
-        static_assert(TW == 64 && GS_ROWS == 64 && 64 % L == 0, "the step phases assume 64 x 64 tiles");
+        static_assert(GS_ROWS == 64 && TW % 64 == 0 && 64 % L == 0, "the step phases assume 64-row tiles");
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
             if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
             const int x0 = me0 + s0, b0 = bme0 + s0;
             float so[L], e[L], bb[L];
-            if (s0 < TW) {
+            if (s0 < 64) {
                 float nh[L], wh[L];
 #pragma unroll
                 for (int q = 0; q < L; ++q) {
@@ -352,6 +353,24 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
                     // xprev is consumed only where this lane was active one
                     // tick earlier (as W) or lane r + 1 is active now (as N,
                     // same column), so it need not be held across inactive ticks
+                    xprev = v;
+                }
+            } else if (s0 < TW) {
+                float nh[L];
+#pragma unroll
+                for (int q = 0; q < L; ++q) {
+                    so[q] = X[x0 + q + XP];
+                    e[q] = X[x0 + q + 1];
+                    bb[q] = Bs[b0 + q];
+                    nh[q] = X[x0 + q - XP];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < L; ++q) {
+                    const float nv = __int_as_float(
+                        __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                    const float v = relax_k<KIND>(bb[q], nv, so[q], xprev, e[q], a, dv);
+                    X[x0 + q] = v;
                     xprev = v;
                 }
             } else {
@@ -736,23 +755,23 @@ void gs_plan_destroy(GsPlan* plan) {
     plan->host_offsets = nullptr;
 }
 
-template <int M, int L>
+template <int TW, int M, int L>
 void launch_unit(hipStream_t s, int n, const int2* w, const SweepSet& set) {
     SweepSet level = set;
     level.units_per_field = n;
     const dim3 grid(n * set.count);
     if (set.a[0].kind == SWEEP_DIFFUSE)
-        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_DIFFUSE>), grid, dim3(64 * M), 0, s, level, w);
+        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_DIFFUSE>), grid, dim3(64 * M), 0, s, level, w);
     else
-        hipLaunchKernelGGL((k_gs_unit<64, M, L, SWEEP_PRESSURE>), grid, dim3(64 * M), 0, s, level, w);
+        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_PRESSURE>), grid, dim3(64 * M), 0, s, level, w);
 }
 
-template <int M>
+template <int TW, int M>
 bool launch_unit_lag(hipStream_t s, int lag, int n, const int2* w, const SweepSet& set) {
     switch (lag) {
-        case 1: launch_unit<M, 1>(s, n, w, set); return true;
-        case 4: launch_unit<M, 4>(s, n, w, set); return true;
-        case 8: launch_unit<M, 8>(s, n, w, set); return true;
+        case 1: launch_unit<TW, M, 1>(s, n, w, set); return true;
+        case 4: launch_unit<TW, M, 4>(s, n, w, set); return true;
+        case 8: launch_unit<TW, M, 8>(s, n, w, set); return true;
         default: return false;
     }
 }
@@ -781,13 +800,20 @@ hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet
             else
                 return hipErrorInvalidValue;
         } else {
-            if (plan.TW != 64) return hipErrorInvalidValue;
             bool ok = false;
-            switch (plan.M) {
-                case 2: ok = launch_unit_lag<2>(s, plan.lag, n, w, set); break;
-                case 4: ok = launch_unit_lag<4>(s, plan.lag, n, w, set); break;
-                case 8: ok = launch_unit_lag<8>(s, plan.lag, n, w, set); break;
-                default: break;
+            if (plan.TW == 64) {
+                switch (plan.M) {
+                    case 2: ok = launch_unit_lag<64, 2>(s, plan.lag, n, w, set); break;
+                    case 4: ok = launch_unit_lag<64, 4>(s, plan.lag, n, w, set); break;
+                    case 8: ok = launch_unit_lag<64, 8>(s, plan.lag, n, w, set); break;
+                    default: break;
+                }
+            } else if (plan.TW == 128) {
+                switch (plan.M) {
+                    case 4: ok = launch_unit_lag<128, 4>(s, plan.lag, n, w, set); break;
+                    case 8: ok = launch_unit_lag<128, 8>(s, plan.lag, n, w, set); break;
+                    default: break;
+                }
             }
             if (!ok) return hipErrorInvalidValue;
         }

@@ -212,6 +212,51 @@ def test_project(R, order, block, gsb, lag, jk, oracle_lib):
         assert_same(gv, wv, f"project v R={R} K={K} order={order}")
 
 
+# 64 x 128 tiles: units of 128 columns (and the one-sweep tile kernel at 128 columns)
+WIDE_VARIANTS = [(8, 4), (8, 1), (8, 8), (4, 4), (4, 8), (1, 0)]
+
+
+@pytest.mark.parametrize("gsb,lag", WIDE_VARIANTS)
+@pytest.mark.parametrize("R", [10, 67, 130, 258, 300])
+def test_wide_tiles_vs_oracle(R, gsb, lag, oracle_lib):
+    """Diffuse (all three boundary rules), projection and whole steps with
+    128-column Gauss-Seidel tiles, bit for bit against the oracle."""
+    st = W.random_state(R, seed=13 * R, vel_scale=1.0)
+
+    def solver(kd=20, kp=20):
+        s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=GS, gs_block=gsb, gs_lag=lag,
+                            gs_tile_cols=128)
+        s.upload(*st[:3])
+        s.set_sources(*st[3:])
+        return s
+
+    for K in (1, 7, 20):
+        with solver() as s:
+            for f in range(3):
+                s.diffuse(f, 5e-2, 0.1, K)
+            got = s.download()
+        for f in range(3):
+            want = st[f].copy()
+            oracle_lib.diffuse(want, f, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert_same(got[f], want, f"wide diffuse R={R} field={f} K={K} gs_block={gsb} lag={lag}")
+    with solver() as s:
+        s.project(20)
+        _, gu, gv = s.download()
+        gp = s.pressure()
+    wu, wv = st[1].copy(), st[2].copy()
+    wp = oracle_lib.project(wu, wv, 20, oracle.GS_LEX, want_p=True)
+    assert_same(gp, wp, f"wide pressure R={R} gs_block={gsb} lag={lag}")
+    assert_same(gu, wu, f"wide project u R={R}")
+    want = [x.copy() for x in st[:3]]
+    with solver(kd=12, kp=30) as s:
+        s.step_n(3, 1e-4, 1e-5, 0.05)
+        got = s.download()
+    for _ in range(3):
+        oracle_lib.solve(want[0], st[3], 1e-4, want[1], want[2], st[4], st[5], 1e-5, 0.05, 12, 30, oracle.GS_LEX)
+    for f, name in enumerate(("density", "u", "v")):
+        assert_same(got[f], want[f], f"wide step {name} R={R} gs_block={gsb} lag={lag}")
+
+
 # ------------------------------------------------ the reference's own outputs
 @pytest.mark.parametrize("R", [4, 6, 10, 18, 34])
 def test_golden_stages(R):

@@ -50,6 +50,7 @@ def test_default_params_are_the_reference_literals():
     (10, 10, {"order": 7}, "sweep_order"),
     (10, 10, {"jacobi_block": 9}, "jacobi_block"),
     (10, 10, {"gs_tile_cols": 48}, "gs_tile_cols"),
+    (10, 10, {"gs_tile_cols": 128, "gs_block": 2}, "gs_tile_cols 128"),
     (10, 10, {"gs_block": 3}, "gs_block"),
     (10, 10, {"gs_lag": 2}, "gs_lag"),
     (10, 10, {"jacobi_kernel": 3}, "jacobi_kernel"),
@@ -172,6 +173,26 @@ def test_gs_unit_ring_as_data_reproduces_gauss_seidel(R, M, L, oracle_lib):
     p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
     p = np.zeros((R, R), np.float32)
     gs_unit_ring_model(p, d, 9, 1, 0, 0, False, False, M, L)
+    assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M,L", [(10, 8, 4), (131, 8, 4), (200, 4, 4), (140, 8, 8)])
+def test_gs_unit_wide_tiles_reproduce_gauss_seidel(R, M, L, oracle_lib):
+    """Units of 64 rows x 128 columns (the three step phases of the plain
+    path): bit-identical to GS-lex, race-free within intervals and levels."""
+    rho, u, v, *_ = W.random_state(R, seed=R + 6)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        x = rho.copy()
+        gs_unit_ring_model(x, rho.copy(), 9, 0, a, den, mode == 2, mode == 1, M, L, TW=128)
+        ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, 9, oracle.GS_LEX)
+        assert bits_equal(x, ref), mode
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
+    p = np.zeros((R, R), np.float32)
+    gs_unit_ring_model(p, d, 9, 1, 0, 0, False, False, M, L, TW=128)
     assert bits_equal(p, p_ref)
 
 
