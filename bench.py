@@ -196,6 +196,27 @@ def cpu_baseline(n: int, kd: int, kp: int, params, sample_n: int):
                       f"count to {n}x{n}; oracle/fs2d_oracle.c, bit-identical to the reference CPU solver"}
 
 
+def run_render(R: int, state, reps: int = 20) -> dict:
+    """Visual output of the resident state (include/fs2d_render.h), R x R cells:
+    density -> RGBA8 (4 B read + 4 B written per cell) and velocity -> line
+    vertices (40 B written per cell, u and v read on every 8th row and column),
+    each into device memory, timed with HIP events on the handle's stream."""
+    import fs2d_amd
+
+    with fs2d_amd.Solver(R, launch_mode=LAUNCH_MODE) as s:
+        s.upload(*state)
+        ms_d, ms_v = s.time_render(reps)
+    cells = R * R
+    bd = 8 * cells
+    bv = 40 * cells + 8 * ((R + 7) // 8) ** 2
+    gd, gv = bd / (ms_d * 1e-3) / 1e9, bv / (ms_v * 1e-3) / 1e9
+    return {"density_rgba8": {"ms": round(ms_d, 4), "gbs_algorithmic": round(gd, 1),
+                              "roofline_frac": round(gd / HBM_PEAK_GBS, 4), "kernel": "k_render_density"},
+            "velocity_lines": {"ms": round(ms_v, 4), "gbs_algorithmic": round(gv, 1),
+                               "roofline_frac": round(gv / HBM_PEAK_GBS, 4), "kernel": "k_render_velocity"},
+            "cells": cells}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +232,7 @@ def main() -> int:
     ap.add_argument("--no-8192", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--launch", choices=("graph", "eager"), default="graph",
                     help="replay captured step graphs (default) or issue every launch")
     ap.add_argument("--cpu-sample-n", type=int, default=2048, help="interior size of the timed CPU sample")
@@ -279,6 +301,8 @@ def main() -> int:
                          "pressure_solve_ms": round(jms, 4),
                          "pressure_solve_gbs_algorithmic": round(12 * cells * K / (jms * 1e-3) / 1e9, 1),
                          "launches_per_step": jl, "kernel": "k_jacobi_wave (depth autotuned in {1,2,4,8})"}
+    if not args.no_render:
+        out["render"] = run_render(R, state)
     del state
     if not args.no_8192 and n == 4096:
         n8, R8 = 8192, 8194

@@ -8,6 +8,7 @@
 // (initial_grid in advect, temp_* before the velocity advection).
 #include "../../include/fs2d_cnn.h"
 #include "../../include/fs2d_hip.h"
+#include "../../include/fs2d_render.h"
 #include "fs2d_internal.hpp"
 
 #include <cmath>
@@ -48,6 +49,8 @@ struct fs2d_handle {
     bool speed_valid = false;   // max_bits holds max |u|, |v| of the current velocity
     long long steps_done = 0;
     double* res_sums = nullptr;  // 2 doubles: residual sums of the seeded projection
+    void* render_buf = nullptr;  // device staging of the visual output, grown on demand
+    size_t render_bytes = 0;
     struct StepGraph;
     std::vector<StepGraph*> graphs;
     std::string err;
@@ -482,6 +485,7 @@ int fs2d_hip_destroy(fs2d_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
     if (h->res_sums) (void)hipFree(h->res_sums);
+    if (h->render_buf) (void)hipFree(h->render_buf);
     for (auto& kv : h->gs_plans) gs_plan_destroy(&kv.second);
     for (float* p : h->f)
         if (p) (void)hipFree(p);
@@ -790,3 +794,84 @@ int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, 
 }
 
 }  // extern "C"
+
+// ---- visual output (src/density_grid_renderer.cu, src/velocity_grid_renderer.cu) ----
+namespace {
+
+int render_staging(fs2d_handle* h, size_t bytes, void** out) {
+    if (h->render_bytes < bytes) {
+        FS2D_HIP(h, hipStreamSynchronize(h->stream));
+        if (h->render_buf) FS2D_HIP(h, hipFree(h->render_buf));
+        h->render_buf = nullptr;
+        h->render_bytes = 0;
+        FS2D_HIP(h, hipMalloc(&h->render_buf, bytes));
+        h->render_bytes = bytes;
+    }
+    *out = h->render_buf;
+    return FS2D_OK;
+}
+
+size_t cell_count(const fs2d_handle* h) { return static_cast<size_t>(h->R) * static_cast<size_t>(h->R); }
+
+}  // namespace
+
+int fs2d_hip_render_density_device(fs2d_handle* h, const fs2d_color_multipliers* m, unsigned char* rgba) {
+    if (!h || !m || !rgba) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_render_density: NULL argument");
+    FS2D_HIP(h, launch_render_density(h->stream, h->rho(), h->R, h->P, m->r, m->g, m->b, rgba));
+    return FS2D_OK;
+}
+
+int fs2d_hip_render_density(fs2d_handle* h, const fs2d_color_multipliers* m, unsigned char* rgba) {
+    if (!h || !m || !rgba) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_render_density: NULL argument");
+    const size_t bytes = 4 * cell_count(h);
+    void* buf = nullptr;
+    int rc = render_staging(h, bytes, &buf);
+    if (rc) return rc;
+    if ((rc = fs2d_hip_render_density_device(h, m, static_cast<unsigned char*>(buf)))) return rc;
+    FS2D_HIP(h, hipMemcpyAsync(rgba, buf, bytes, hipMemcpyDeviceToHost, h->stream));
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    return FS2D_OK;
+}
+
+int fs2d_hip_render_velocity_device(fs2d_handle* h, float hs, float vs, fs2d_line* lines) {
+    static_assert(sizeof(fs2d_vertex) == 20 && sizeof(fs2d_line) == 40, "SFML vertex layout");
+    if (!h || !lines) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_render_velocity: NULL argument");
+    FS2D_HIP(h, launch_render_velocity(h->stream, h->u(), h->v(), h->R, h->P, hs, vs, lines));
+    return FS2D_OK;
+}
+
+int fs2d_hip_render_velocity(fs2d_handle* h, float hs, float vs, fs2d_line* lines) {
+    if (!h || !lines) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_render_velocity: NULL argument");
+    const size_t bytes = sizeof(fs2d_line) * cell_count(h);
+    void* buf = nullptr;
+    int rc = render_staging(h, bytes, &buf);
+    if (rc) return rc;
+    if ((rc = fs2d_hip_render_velocity_device(h, hs, vs, static_cast<fs2d_line*>(buf)))) return rc;
+    FS2D_HIP(h, hipMemcpyAsync(lines, buf, bytes, hipMemcpyDeviceToHost, h->stream));
+    FS2D_HIP(h, hipStreamSynchronize(h->stream));
+    return FS2D_OK;
+}
+
+int fs2d_hip_time_render(fs2d_handle* h, int reps, float* ms_density, float* ms_velocity) {
+    if (!h || reps < 1 || !ms_density || !ms_velocity) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_time_render: bad argument");
+    void* buf = nullptr;
+    int rc = render_staging(h, sizeof(fs2d_line) * cell_count(h), &buf);
+    if (rc) return rc;
+    const fs2d_color_multipliers m{1.f, 0.5f, 0.25f};
+    for (int pass = 0; pass < 2; ++pass) {
+        auto once = [&]() {
+            return pass == 0 ? fs2d_hip_render_density_device(h, &m, static_cast<unsigned char*>(buf))
+                             : fs2d_hip_render_velocity_device(h, 1.f, 1.f, static_cast<fs2d_line*>(buf));
+        };
+        if ((rc = once())) return rc;  // warm
+        FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
+        for (int r = 0; r < reps; ++r)
+            if ((rc = once())) return rc;
+        FS2D_HIP(h, hipEventRecord(h->ev1, h->stream));
+        FS2D_HIP(h, hipEventSynchronize(h->ev1));
+        float ms = 0.f;
+        FS2D_HIP(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        (pass == 0 ? *ms_density : *ms_velocity) = ms / reps;
+    }
+    return FS2D_OK;
+}

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include <cstdint>
 
@@ -107,6 +108,13 @@ hipError_t launch_gradient(hipStream_t s, float* u, float* v, const float* p, in
 hipError_t launch_set_boundary(hipStream_t s, float* x, int R, int P, Ring ring);
 hipError_t launch_copy4(hipStream_t s, float4* dst, const float4* src, size_t n4);
 hipError_t launch_fill(hipStream_t s, void* p, uint32_t value, size_t n_words);
+
+// Visual output (fs2d_render.hip): density -> RGBA8 (R * R * 4 bytes), velocity
+// -> line vertices (R * R lines of 40 bytes), both unpitched, row-major.
+hipError_t launch_render_density(hipStream_t s, const float* d, int R, int P, float mr, float mg, float mb,
+                                 void* rgba);
+hipError_t launch_render_velocity(hipStream_t s, const float* u, const float* v, int R, int P, float hs, float vs,
+                                  void* lines);
 
 // Gauss-Seidel tile wavefront: one launch per level of the tile hyperplane.
 struct GsPlan {

@@ -42,6 +42,16 @@ class BatchSim(ctypes.Structure):
 _lib = None
 
 
+class ColorMultipliers(ctypes.Structure):
+    """fs2d_color_multipliers (include/fs2d_render.h)."""
+    _fields_ = [("r", _f), ("g", _f), ("b", _f)]
+
+
+# fs2d_vertex / fs2d_line (include/fs2d_render.h): an SFML vertex and a line of two.
+VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("color", "u1", (4,)), ("tex", "<f4", (2,))])
+LINE = np.dtype([("start", VERTEX), ("end", VERTEX)])
+
+
 def library() -> ctypes.CDLL:
     """Load libfs2d_hip.so (raises if it has not been built)."""
     global _lib
@@ -88,6 +98,11 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_launches_per_step": (_i, [H, ctypes.POINTER(_i)]),
         "fs2d_hip_last_error": (ctypes.c_char_p, [H]),
         "fs2d_hip_abi_version": (_i, []),
+        "fs2d_hip_render_density": (_i, [H, ctypes.POINTER(ColorMultipliers), _p]),
+        "fs2d_hip_render_density_device": (_i, [H, ctypes.POINTER(ColorMultipliers), _p]),
+        "fs2d_hip_render_velocity": (_i, [H, _f, _f, _p]),
+        "fs2d_hip_render_velocity_device": (_i, [H, _f, _f, _p]),
+        "fs2d_hip_time_render": (_i, [H, _i, ctypes.POINTER(_f), ctypes.POINTER(_f)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -220,6 +235,27 @@ class Solver:
         self._ck(self._L.fs2d_hip_time_pressure_sweeps(self._h, reps, ctypes.byref(ms)))
         return float(ms.value)
 
+    def render_density(self, r: float, g: float, b: float) -> np.ndarray:
+        """The density as (R, R, 4) uint8 RGBA (density_grid_renderer's kernel,
+        src/density_grid_renderer.cu:10-29)."""
+        out = np.empty((self.R, self.R, 4), np.uint8)
+        m = ColorMultipliers(r, g, b)
+        self._ck(self._L.fs2d_hip_render_density(self._h, ctypes.byref(m), out.ctypes.data))
+        return out
+
+    def render_velocity(self, horizontal_scale: float, vertical_scale: float) -> np.ndarray:
+        """The velocity as an (R, R) array of LINE records (velocity_grid_renderer's
+        kernel, src/velocity_grid_renderer.cu:8-44)."""
+        out = np.empty((self.R, self.R), LINE)
+        self._ck(self._L.fs2d_hip_render_velocity(self._h, horizontal_scale, vertical_scale, out.ctypes.data))
+        return out
+
+    def time_render(self, reps: int = 10):
+        """Mean ms of the density and the velocity conversion (device output)."""
+        a, b = _f(), _f()
+        self._ck(self._L.fs2d_hip_time_render(self._h, reps, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def launches_per_step(self) -> int:
         n = _i()
         self._ck(self._L.fs2d_hip_launches_per_step(self._h, ctypes.byref(n)))
@@ -241,6 +277,8 @@ EXPORTED_SYMBOLS = (
     "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",
     "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_launches_per_step",
     "fs2d_hip_last_error", "fs2d_hip_abi_version",
+    "fs2d_hip_render_density", "fs2d_hip_render_density_device", "fs2d_hip_render_velocity",
+    "fs2d_hip_render_velocity_device", "fs2d_hip_time_render",
 )
 
 

@@ -35,6 +35,11 @@ _i = ctypes.c_int
 _p = ctypes.c_void_p
 
 
+# One SFML vertex (position, RGBA color, texture coordinates) and a line of two.
+VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("color", "u1", (4,)), ("tex", "<f4", (2,))])
+LINE = np.dtype([("start", VERTEX), ("end", VERTEX)])
+
+
 def _ptr(a: np.ndarray) -> int:
     assert a.flags.c_contiguous
     return a.ctypes.data
@@ -70,6 +75,8 @@ class Oracle(_Lib):
         L.fso_poisson_residual_f64.argtypes = [_p, _p, _i]
         L.fso_poisson_residual_f64.restype = ctypes.c_double
         L.fso_divergence.argtypes = [_p, _p, _p, _i]
+        L.fso_render_density.argtypes = [_p, _i, _i, _f, _f, _f, _p]
+        L.fso_render_velocity.argtypes = [_p, _p, _i, _i, _f, _f, _p]
 
     # each call works in place on its array arguments
     def set_boundary(self, x, mode):
@@ -121,6 +128,21 @@ class Oracle(_Lib):
     def poisson_residual_f64(self, d, p) -> float:
         R = d.shape[0]; _check(d, R); _check(p, R)
         return float(self.lib.fso_poisson_residual_f64(_ptr(d), _ptr(p), R))
+
+    def render_density(self, grid, mr, mg, mb):
+        """(R, R, 4) uint8 RGBA of a density grid (src/density_grid_renderer.cu:10-29)."""
+        R = grid.shape[0]; _check(grid, R)
+        out = np.zeros((R, R, 4), np.uint8)
+        self.lib.fso_render_density(_ptr(grid), R, R, mr, mg, mb, _ptr(out))
+        return out
+
+    def render_velocity(self, u, v, hs, vs):
+        """(R, R) array of LINE records (start and end vertex) of a velocity grid
+        (src/velocity_grid_renderer.cu:8-44)."""
+        R = u.shape[0]; _check(u, R); _check(v, R)
+        out = np.zeros((R, R), LINE)
+        self.lib.fso_render_velocity(_ptr(u), _ptr(v), R, R, hs, vs, _ptr(out))
+        return out
 
 
 class Reference(_Lib):

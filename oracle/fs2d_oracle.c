@@ -304,3 +304,41 @@ double fso_poisson_residual_f64(const float* d, const float* p, int R) {
         }
     return den > 0.0 ? sqrt(num / den) : sqrt(num);
 }
+
+/* ---- visual output ------------------------------------------------------ */
+
+/* src/density_grid_renderer.cu:17-26.  The reference's device max/min on
+ * floats are fmaxf/fminf; static_cast<uint8_t> truncates a value in [0, 255]. */
+void fso_render_density(const float* grid, int rows, int cols, float mr, float mg, float mb, unsigned char* rgba) {
+    for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j) {
+            const float d = grid[(size_t)i * cols + j];
+            unsigned char* px = rgba + 4 * ((size_t)i * cols + j);
+            px[0] = (unsigned char)fmaxf(0.f, fminf(255.f, mr * d));
+            px[1] = (unsigned char)fmaxf(0.f, fminf(255.f, mg * d));
+            px[2] = (unsigned char)fmaxf(0.f, fminf(255.f, mb * d));
+            px[3] = 255;
+        }
+}
+
+/* src/velocity_grid_renderer.cu:17-41: the size_t operands convert to float
+ * (j and i exactly; rows * cols rounds to float above 2^24 before sqrtf, as in
+ * the reference). */
+void fso_render_velocity(const float* u, const float* v, int rows, int cols, float hs, float vs, fso_vertex* lines) {
+    const float s = sqrtf((float)((size_t)rows * (size_t)cols));
+    for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j) {
+            fso_vertex* ln = lines + 2 * ((size_t)i * cols + j);
+            const float x = (float)j * hs, y = (float)i * vs;
+            float ex = x, ey = y;
+            if (i % 8 == 0 && j % 8 == 0) {
+                const float vx = u[(size_t)i * cols + j], vy = v[(size_t)i * cols + j];
+                ex = ex + 250000.f * vx / s;
+                ey = ey + 250000.f * vy / s;
+            }
+            const fso_vertex a = {x, y, 255, 255, 255, 255, 0.f, 0.f};
+            const fso_vertex b = {ex, ey, 255, 255, 255, 255, 0.f, 0.f};
+            ln[0] = a;
+            ln[1] = b;
+        }
+}

@@ -55,6 +55,25 @@ double fso_poisson_residual_f64(const float* d, const float* p, int R);
 /* Divergence field as the reference computes it (float, interior only). */
 void fso_divergence(float* d, const float* u, const float* v, int R);
 
+/* Visual output (the renderers of the reference's GPU build) -------------- */
+/* src/density_grid_renderer.cu:10-29: every cell of the rows x cols grid to an
+ * RGBA8 pixel, channel c = (uint8) max(0, min(255, m_c * d)) with C99
+ * fminf/fmaxf (a NaN product clamps to 255), alpha 255. */
+void fso_render_density(const float* grid, int rows, int cols, float mr, float mg, float mb, unsigned char* rgba);
+
+/* One SFML vertex: position, RGBA color, texture coordinates (20 bytes). */
+typedef struct fso_vertex {
+    float x, y;
+    unsigned char r, g, b, a;
+    float tx, ty;
+} fso_vertex;
+
+/* src/velocity_grid_renderer.cu:8-44: every cell (i, j) to a line (start, end);
+ * start = end = (j * hs, i * vs); where i % 8 == 0 and j % 8 == 0 the end moves
+ * by 250000 * (u, v) / sqrtf(rows * cols); colors white, texture coordinates 0
+ * (the reference clears the vertex buffer first). */
+void fso_render_velocity(const float* u, const float* v, int rows, int cols, float hs, float vs, fso_vertex* lines);
+
 #ifdef __cplusplus
 }
 #endif

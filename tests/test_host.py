@@ -20,7 +20,7 @@ from schedule_models import (gs_tile_model, gs_unit_lag_model, gs_unit_model, gs
 
 
 def header_symbols():
-    text = "".join(open(os.path.join(ROOT, "include", f)).read() for f in ("fs2d_hip.h", "fs2d_cnn.h"))
+    text = "".join(open(os.path.join(ROOT, "include", f)).read() for f in ("fs2d_hip.h", "fs2d_cnn.h", "fs2d_render.h"))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(fs2d_[a-z0-9_]+)\s*\(", text)))
 

@@ -278,3 +278,56 @@ def test_reference_denies_nan_scatter_is_documented(oracle_lib):
     u = st[1].copy(); u[4, 4] = np.nan
     y = st[0].copy(); oracle_lib.advect(y, u, st[2], 0, 0.1, True)
     assert np.isfinite(y).all()
+
+
+# ------------------------------------------------ visual output (renderers)
+def test_render_density_known_answers(oracle_lib):
+    """Pixels of src/density_grid_renderer.cu:17-26 worked by hand: channel =
+    trunc(clamp(m * d, 0, 255)), a NaN product clamps to 255 (the device
+    fminf/fmaxf return the non-NaN operand), alpha 255."""
+    d = np.array([[0.5, 1.0, 2.0, -1.0], [np.nan, np.inf, -np.inf, -0.0],
+                  [np.float32(1) / np.float32(255), 0.1, 0.25, 100.0], [0.0, 3.0, 0.125, 1e-30]], np.float32)
+    px = oracle_lib.render_density(d, 255.0, 128.0, 1.0)
+    want_r = [[127, 255, 255, 0], [255, 255, 0, 0], [1, 25, 63, 255], [0, 255, 31, 0]]
+    want_g = [[64, 128, 255, 0], [255, 255, 0, 0], [0, 12, 32, 255], [0, 255, 16, 0]]
+    want_b = [[0, 1, 2, 0], [255, 255, 0, 0], [0, 0, 0, 100], [0, 3, 0, 0]]
+    assert np.array_equal(px[:, :, 0], want_r)
+    assert np.array_equal(px[:, :, 1], want_g)
+    assert np.array_equal(px[:, :, 2], want_b)
+    assert (px[:, :, 3] == 255).all()
+
+
+def test_render_velocity_known_answers(oracle_lib):
+    """Lines of src/velocity_grid_renderer.cu:17-41 on a 64 x 64 grid, where
+    sqrtf(rows * cols) = 64 exactly: with u = 0.5 and v = -0.25 every 8th row
+    and column ends 250000 * 0.5 / 64 = 1953.125 right and 976.5625 up of its
+    start; the others are points; all white, texture coordinates 0."""
+    R = 64
+    u = np.full((R, R), 0.5, np.float32)
+    v = np.full((R, R), -0.25, np.float32)
+    L = oracle_lib.render_velocity(u, v, 2.0, 3.0)
+    i, j = np.meshgrid(np.arange(R), np.arange(R), indexing="ij")
+    assert np.array_equal(L["start"]["x"], (2.0 * j).astype(np.float32))
+    assert np.array_equal(L["start"]["y"], (3.0 * i).astype(np.float32))
+    moved = (i % 8 == 0) & (j % 8 == 0)
+    assert np.array_equal(L["end"]["x"], np.where(moved, 2.0 * j + 1953.125, 2.0 * j).astype(np.float32))
+    assert np.array_equal(L["end"]["y"], np.where(moved, 3.0 * i - 976.5625, 3.0 * i).astype(np.float32))
+    for end in ("start", "end"):
+        assert (L[end]["color"] == 255).all() and (L[end]["tex"] == 0).all()
+    # a zero velocity leaves every line a point
+    Z = oracle_lib.render_velocity(np.zeros((R, R), np.float32), np.zeros((R, R), np.float32), 1.5, 0.5)
+    assert np.array_equal(Z["start"]["x"], Z["end"]["x"]) and np.array_equal(Z["start"]["y"], Z["end"]["y"])
+
+
+def test_render_velocity_length_and_direction(oracle_lib):
+    """At the sampled cells the end offset is the velocity scaled by
+    250000 / sqrt(rows * cols), within float rounding of the two operations."""
+    R = 130
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((R, R)).astype(np.float32) * 1e-3
+    v = rng.standard_normal((R, R)).astype(np.float32) * 1e-3
+    L = oracle_lib.render_velocity(u, v, 1.0, 1.0)
+    s = np.sqrt(np.float64(R) * R)
+    dx = L["end"]["x"][::8, ::8].astype(np.float64) - L["start"]["x"][::8, ::8]
+    want = 250000.0 * u[::8, ::8].astype(np.float64) / s
+    assert np.allclose(dx, want, rtol=1e-5, atol=2e-4)
