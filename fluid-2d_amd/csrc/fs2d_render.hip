@@ -28,26 +28,40 @@ __global__ __launch_bounds__(kRenderThreads) void k_render_density(const float* 
 
 // One line per cell: start and end vertex of 5 words each (x, y, RGBA, texture
 // x, texture y).  start = end = (j hs, i vs); on every 8th row and column the
-// end follows the velocity by 250000 (u, v) / s, s = sqrtf(rows * cols).
+// end follows the velocity by 250000 (u, v) / s, s = sqrtf(rows * cols).  The
+// block's lines (256 cells of one row, 10240 contiguous bytes) are assembled in
+// LDS and written as consecutive 8-byte words, so each store instruction
+// covers 512 contiguous bytes.
 __global__ __launch_bounds__(kRenderThreads) void k_render_velocity(const float* __restrict__ u,
                                                                     const float* __restrict__ v, int R, int P,
                                                                     float hs, float vs, float s,
                                                                     float2* __restrict__ lines) {
-    const int i = blockIdx.y, j = blockIdx.x * kRenderThreads + threadIdx.x;
-    if (j >= R) return;
-    const float x = static_cast<float>(j) * hs, y = static_cast<float>(i) * vs;
-    float ex = x, ey = y;
-    if ((i & 7) == 0 && (j & 7) == 0) {
-        ex = ex + 250000.f * u[(size_t)i * P + j] / s;
-        ey = ey + 250000.f * v[(size_t)i * P + j] / s;
+    constexpr int W = 5;  // 8-byte words per line
+    __shared__ float2 stage[kRenderThreads * W];
+    const int i = blockIdx.y, j0 = blockIdx.x * kRenderThreads, j = j0 + threadIdx.x;
+    if (j < R) {
+        const float x = static_cast<float>(j) * hs, y = static_cast<float>(i) * vs;
+        float ex = x, ey = y;
+        if ((i & 7) == 0 && (j & 7) == 0) {
+            ex = ex + 250000.f * u[(size_t)i * P + j] / s;
+            ey = ey + 250000.f * v[(size_t)i * P + j] / s;
+        }
+        const float white = __uint_as_float(0xffffffffu);
+        float2* o = stage + W * threadIdx.x;
+        o[0] = make_float2(x, y);
+        o[1] = make_float2(white, 0.f);
+        o[2] = make_float2(0.f, ex);
+        o[3] = make_float2(ey, white);
+        o[4] = make_float2(0.f, 0.f);
     }
-    const float white = __uint_as_float(0xffffffffu);
-    float2* o = lines + 5 * ((size_t)i * R + j);
-    o[0] = make_float2(x, y);
-    o[1] = make_float2(white, 0.f);
-    o[2] = make_float2(0.f, ex);
-    o[3] = make_float2(ey, white);
-    o[4] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int words = W * min(kRenderThreads, R - j0);
+    float2* dst = lines + (size_t)W * ((size_t)i * R + j0);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int w = threadIdx.x + k * kRenderThreads;
+        if (w < words) dst[w] = stage[w];
+    }
 }
 
 dim3 render_grid(int R) { return dim3((R + kRenderThreads - 1) / kRenderThreads, R); }
