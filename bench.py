@@ -259,7 +259,7 @@ def main() -> int:
     solve_bytes = 12 * cells * K  # read p, read d, write p per cell per sweep
     solve_gbs = solve_bytes / (ms_solve * 1e-3) / 1e9
     out = {
-        "metric": "steps/s (plain stable-fluids step, 4096^2 interior, K=80)",
+        "metric": f"steps/s (plain stable-fluids step, {n}^2 interior, K={K})",
         "value": round(1.0 / sec_step, 3),
         "unit": "steps/s",
         "n_gpus": 1,

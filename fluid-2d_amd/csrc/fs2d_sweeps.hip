@@ -20,12 +20,16 @@
 //    a full-wave DPP shift; south, east and the right-hand side come from LDS.
 //    The ring of sweep k-1 (k >= 1) is the sign-adjusted adjacent interior
 //    cell of sweep k-1, so it is never stored until the last sweep writes it;
-//    sweep 0 reads the stored (stale) ring as the reference does.
+//    sweep 0 reads the stored (stale) ring as the reference does.  The
+//    default kernel, k_gs_unit, blocks M sweeps per tile in skewed
+//    coordinates (one wave per sweep, lagged; see its comment).
 //
 //  * Jacobi -- every neighbour from the previous iterate.  M sweeps are fused
-//    per launch (temporal blocking): a 256-thread block loads its tile plus an
-//    M-cell halo into LDS once, relaxes M times in LDS on a shrinking region
-//    (recomputing the ring in LDS after each sweep), and writes only the tile.
+//    per launch (temporal blocking).  The default kernel, k_jacobi_wave, keeps
+//    a tile's rows in registers, one wave per tile (fs2d_jacobi.hpp); the
+//    alternative, k_jacobi_block, has a 256-thread block load its tile plus an
+//    M-cell halo into LDS once, relax M times in LDS on a shrinking region
+//    (recomputing the ring in LDS after each sweep), and write only the tile.
 #include <type_traits>
 
 #include "fs2d_internal.hpp"
@@ -544,8 +548,6 @@ __global__ __launch_bounds__(JT) void k_jacobi_block(SweepArgs A, const float* _
     const Divisor dv = divisor_of(A);
 #pragma unroll
     for (int s = 1; s <= M; ++s) {
-        constexpr int dummy = 0;
-        (void)dummy;
         const int e = M - s;  // the region relaxed by sweep s is the tile grown by e
         const int rw = JX + 2 * e, rh = JY + 2 * e;
         const float* cur = buf[(s - 1) & 1];
