@@ -51,6 +51,11 @@ struct fs2d_handle {
     double* res_sums = nullptr;  // 2 doubles: residual sums of the seeded projection
     void* render_buf = nullptr;  // device staging of the visual output, grown on demand
     size_t render_bytes = 0;
+    struct MgLevel {
+        int n, R, P;
+        float *p, *t, *b;  // iterate, its ping-pong partner, right-hand side
+    };
+    std::vector<MgLevel> mg;  // coarse grids of the multigrid seed (side n/2, n/4, ...), on first use
     struct StepGraph;
     std::vector<StepGraph*> graphs;
     std::string err;
@@ -486,6 +491,9 @@ int fs2d_hip_destroy(fs2d_handle* h) {
     drop_graphs(h);
     if (h->res_sums) (void)hipFree(h->res_sums);
     if (h->render_buf) (void)hipFree(h->render_buf);
+    for (auto& L : h->mg)
+        for (float* q : {L.p, L.t, L.b})
+            if (q) (void)hipFree(q);
     for (auto& kv : h->gs_plans) gs_plan_destroy(&kv.second);
     for (float* p : h->f)
         if (p) (void)hipFree(p);
@@ -751,9 +759,59 @@ static int jacobi_until(fs2d_handle* h, float tol, int max_sweeps, int check, in
     return FS2D_OK;
 }
 
+// ---- multigrid seed: V(2,2) cycles of weighted Jacobi on grids of side n, n/2, ...
+constexpr float kMgOmega = 0.8f;
+constexpr int kMgPre = 2, kMgPost = 2, kMgMaxCycles = 30;
+
+// Sweeps on the coarsest grid: its slowest mode decays as 1 - omega pi^2 / (2 n^2)
+// per sweep, so n^2 sweeps reduce it by ~50x (64 at least, 4096 at most).
+static int mg_coarsest_sweeps(int n) { return std::min(4096, std::max(64, n * n)); }
+
+static int mg_levels(fs2d_handle* h) {
+    if (!h->mg.empty()) return FS2D_OK;
+    int n = h->R - 2;
+    while (n % 2 == 0 && n / 2 >= 4) {
+        n /= 2;
+        fs2d_handle::MgLevel L{n, n + 2, (n + 2 + 63) / 64 * 64, nullptr, nullptr, nullptr};
+        const size_t bytes = sizeof(float) * static_cast<size_t>(L.R) * L.P;
+        h->mg.push_back(L);  // owned by the handle from here on, freed in destroy
+        fs2d_handle::MgLevel& M = h->mg.back();
+        FS2D_HIP(h, hipMalloc(&M.p, bytes));
+        FS2D_HIP(h, hipMalloc(&M.t, bytes));
+        FS2D_HIP(h, hipMalloc(&M.b, bytes));
+        FS2D_HIP(h, launch_fill(h->stream, M.b, 0u, bytes / 4));
+    }
+    return FS2D_OK;
+}
+
+// One V-cycle on level l (0: the handle's pressure, scratch and divergence
+// fields); the result is left in the level's iterate pointer.
+static int mg_vcycle(fs2d_handle* h, size_t l) {
+    float** p = l == 0 ? &h->f[F_P] : &h->mg[l - 1].p;
+    float** t = l == 0 ? &h->f[F_SCRATCH] : &h->mg[l - 1].t;
+    const float* b = l == 0 ? h->f[F_D] : h->mg[l - 1].b;
+    const int R = l == 0 ? h->R : h->mg[l - 1].R, P = l == 0 ? h->P : h->mg[l - 1].P;
+    auto smooth = [&](int count) -> int {
+        for (int k = 0; k < count; ++k) {
+            FS2D_HIP(h, launch_mg_smooth(h->stream, *p, *t, b, R, P, kMgOmega));
+            std::swap(*p, *t);
+        }
+        return FS2D_OK;
+    };
+    int rc;
+    if (l == h->mg.size()) return smooth(mg_coarsest_sweeps(R - 2));
+    if ((rc = smooth(kMgPre))) return rc;
+    fs2d_handle::MgLevel& C = h->mg[l];
+    FS2D_HIP(h, launch_mg_restrict(h->stream, *p, b, P, C.b, C.R, C.P));
+    FS2D_HIP(h, launch_fill(h->stream, C.p, 0u, static_cast<size_t>(C.R) * C.P));
+    if ((rc = mg_vcycle(h, l + 1))) return rc;
+    FS2D_HIP(h, launch_mg_prolong(h->stream, C.p, C.P, *p, R, P));
+    return smooth(kMgPost);
+}
+
 int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, int max_sweeps, int check,
                             int fallback, int* sweeps_run, double* rel_residual, int* fell_back) {
-    if (!h || seed < 0 || seed > 2 || max_sweeps < 0 || check < 1 || !(tol >= 0.f))
+    if (!h || seed < 0 || seed > 3 || max_sweeps < 0 || check < 1 || !(tol >= 0.f))
         return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_project_seeded: bad argument");
     if (h->prm.sweep_order != FS2D_ORDER_JACOBI)
         return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_project_seeded: needs the Jacobi order");
@@ -762,7 +820,7 @@ int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, 
                     "fs2d_hip_project_seeded: seed 2 needs a network and a side n = rows - 2 that is a multiple of 32");
     const Scalars s = scalars(h->R, 1.0f);
     FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h, nullptr));
-    if (seed == 0 || h->pressure_zero) FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
+    if (seed == 0 || seed == 3 || h->pressure_zero) FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
     const bool keep_prev = seed == 2 && fallback;
     if (keep_prev)  // the previous pressure, for the fallback
         FS2D_HIP(h, hipMemcpyAsync(h->f[F_T1], h->f[F_P], h->field_bytes, hipMemcpyDeviceToDevice, h->stream));
@@ -773,7 +831,18 @@ int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, 
     }
     int sweeps = 0, rc;
     double rel = 0;
+    int fine_smooth = 0;  // weighted Jacobi sweeps of the fine grid in V-cycles
+    if (seed == 3) {
+        if ((rc = mg_levels(h))) return rc;
+        for (int c = 0; c < kMgMaxCycles; ++c) {
+            if ((rc = mg_vcycle(h, 0))) return rc;
+            fine_smooth += h->mg.empty() ? mg_coarsest_sweeps(h->R - 2) : kMgPre + kMgPost;
+            if ((rc = residual_of(h, &rel))) return rc;
+            if (rel <= tol) break;
+        }
+    }
     if ((rc = jacobi_until(h, tol, max_sweeps, check, &sweeps, &rel))) return rc;
+    sweeps += fine_smooth;
     int fb = 0;
     if (keep_prev && !(rel <= tol)) {
         FS2D_HIP(h, hipMemcpyAsync(h->f[F_P], h->f[F_T1], h->field_bytes, hipMemcpyDeviceToDevice, h->stream));

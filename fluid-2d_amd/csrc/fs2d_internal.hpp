@@ -116,6 +116,16 @@ hipError_t launch_render_density(hipStream_t s, const float* d, int R, int P, fl
 hipError_t launch_render_velocity(hipStream_t s, const float* u, const float* v, int R, int P, float hs, float vs,
                                   void* lines);
 
+// Multigrid pieces for the pressure equation with the continuous ring
+// (fs2d_mg.hip).  smooth: one weighted Jacobi sweep src -> dst.  restrict: the
+// coarse right-hand side of each coarse interior cell = the sum of the
+// residuals b - (4p - sum nbrs) of its 2 x 2 fine cells (fine side = 2 x coarse
+// side).  prolong: pf += bilinear interpolation of pc (pc's ring included).
+// Each writes the ring of what it writes.
+hipError_t launch_mg_smooth(hipStream_t s, const float* src, float* dst, const float* b, int R, int P, float omega);
+hipError_t launch_mg_restrict(hipStream_t s, const float* p, const float* b, int Pf, float* bc, int Rc, int Pc);
+hipError_t launch_mg_prolong(hipStream_t s, const float* pc, int Pc, float* pf, int Rf, int Pf);
+
 // Gauss-Seidel tile wavefront: one launch per level of the tile hyperplane.
 struct GsPlan {
     int TW;                // tile columns

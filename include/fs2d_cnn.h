@@ -71,14 +71,20 @@ const char* fs2d_net_last_error(const fs2d_net* net);
  * of zero:
  *   seed = 0: zero (the reference's start), 1: the previous pressure (the
  *   handle's pressure field as the last projection left it), 2: the network's
- *   prediction from (divergence, previous pressure) (net required).
+ *   prediction from (divergence, previous pressure) (net required), 3:
+ *   multigrid from zero -- V(2,2) cycles of weighted Jacobi (omega 0.8) on
+ *   grids of side n, n/2, ... down to a side of 4 or an odd side (n_c^2
+ *   sweeps on the coarsest, 64 to 4096), residuals restricted by 2 x 2 sums,
+ *   corrections interpolated bilinearly, until tol or 30 cycles (net
+ *   ignored).
  * Jacobi sweeps run in launches of the handle's depth until the compact
  * Poisson residual ||d - L p||_2 <= tol * ||d||_2 (checked every `check`
  * sweeps, rounded up to whole launches) or max_sweeps is reached; then the
  * gradient is applied.  With seed = 2 and fallback != 0, a seeded solve that
  * misses tol within max_sweeps restarts from the previous pressure (seed 1)
  * with the same budget.  Reports the sweeps run, the final relative residual
- * and whether the fallback ran.  Requires the Jacobi order and a grid side
+ * and whether the fallback ran (sweeps_run counts the fine grid's sweeps,
+ * V-cycle smoothing included).  Requires the Jacobi order and a grid side
  * n = rows - 2 that is a multiple of 32 for seed 2. */
 struct fs2d_handle;
 int fs2d_hip_project_seeded(struct fs2d_handle* h, fs2d_net* net, int seed, float tol, int max_sweeps, int check,
