@@ -470,29 +470,40 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     }
     __syncthreads();
 
-    // ---- write back every cell this unit relaxed (its latest sweep).  Edge
-    // cells of a unit that meets the ring also write the ring they border
-    // (after the solve's last sweep that is the final ring).
+    // ---- write back every cell this unit relaxed (its latest sweep).  A unit
+    // that meets the ring then writes the ring beside each of its edge cells
+    // (after the solve's last sweep that is the final ring), from LDS.
+    auto relaxed = [&](int gi, int gj) {
+        const int lo = max(k_lo, max(GS_ROWS * I + 1 - gi, TW * J + 1 - gj));
+        const int hi = min(k_hi, min(GS_ROWS * I + GS_ROWS - gi, TW * J + TW - gj));
+        return gi >= 1 && gi <= n && gj >= 1 && gj <= n && lo <= hi;
+    };
     for (int idx = tid; idx < XH * XW; idx += NT) {
         const int li = idx / XW, lj = idx % XW;
         const int gi = gi_lo + li, gj = gj_lo + lj;
-        const int lo = max(k_lo, max(GS_ROWS * I + 1 - gi, TW * J + 1 - gj));
-        const int hi = min(k_hi, min(GS_ROWS * I + GS_ROWS - gi, TW * J + TW - gj));
-        if (gi < 1 || gi > n || gj < 1 || gj > n || lo > hi) continue;
-        const float v = X[li * XP + lj];
-        const size_t o = (size_t)gi * P + gj;
-        x[o] = v;
-        if (!plain) {  // the ring of the cell's latest sweep: what the next unit reading it expects
-            const float tb = sgn(rg.neg_tb, v), lr = sgn(rg.neg_lr, v);
-            const float cn = 0.5f * (tb + lr);
-            if (gi == 1) x[o - P] = tb;
-            if (gi == n) x[o + P] = tb;
-            if (gj == 1) x[o - 1] = lr;
-            if (gj == n) x[o + 1] = lr;
-            if (gi == 1 && gj == 1) x[o - P - 1] = cn;
-            if (gi == 1 && gj == n) x[o - P + 1] = cn;
-            if (gi == n && gj == 1) x[o + P - 1] = cn;
-            if (gi == n && gj == n) x[o + P + 1] = cn;
+        if (relaxed(gi, gj)) x[(size_t)gi * P + gj] = X[li * XP + lj];
+    }
+    if (!plain) {
+        // rows 1 and n, then columns 1 and n, of this unit's region
+        for (int e = tid; e < 2 * XW + 2 * XH; e += NT) {
+            const bool row = e < 2 * XW;
+            const int t = row ? e % XW : (e - 2 * XW) % XH;
+            const bool far = row ? e >= XW : e - 2 * XW >= XH;
+            const int gi = row ? (far ? n : 1) : gi_lo + t;
+            const int gj = row ? gj_lo + t : (far ? n : 1);
+            if (!relaxed(gi, gj)) continue;
+            const float v = X[(gi - gi_lo) * XP + (gj - gj_lo)];
+            const size_t o = (size_t)gi * P + gj;
+            if (row) {
+                const float tb = sgn(rg.neg_tb, v);
+                const size_t ring = far ? o + P : o - P;
+                x[ring] = tb;
+                const float cn = 0.5f * (tb + sgn(rg.neg_lr, v));  // the corners beside it
+                if (gj == 1) x[ring - 1] = cn;
+                if (gj == n) x[ring + 1] = cn;
+            } else {
+                x[far ? o + 1 : o - 1] = sgn(rg.neg_lr, v);
+            }
         }
     }
 }
