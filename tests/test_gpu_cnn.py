@@ -234,3 +234,20 @@ def test_multigrid_seed_cycles(R):
             _, u, v = s.download()
         assert rel <= tol and sw <= 4 * max_cycles, (R, tol, sw, rel)
         assert np.isfinite(u).all() and np.isfinite(v).all()
+
+
+@pytest.mark.parametrize("R", [3, 4, 10])
+def test_multigrid_seed_tiny_grids(R):
+    """Seed 3 where no grid can be coarsened (n = 1, 2) or only once (n = 8):
+    it smooths on the grid it has, finishes, and leaves finite fields."""
+    rng = np.random.default_rng(R)
+    rho = rng.random((R, R), dtype=np.float32)
+    u = (rng.standard_normal((R, R)) * 0.01).astype(np.float32)
+    v = (rng.standard_normal((R, R)) * 0.01).astype(np.float32)
+    with fs2d_amd.Solver(R, diffuse_iterations=4, pressure_iterations=4, order=fs2d_amd.JACOBI) as s:
+        s.upload(rho, u, v)
+        sw, rel, fb, _ = PN.seeded_projection(s, None, 3, 1e-3, 64, check=8, fallback=False)
+        _, u1, v1 = s.download()
+        p = s.pressure()
+    assert sw >= 1 and np.isfinite(rel)
+    assert np.isfinite(u1).all() and np.isfinite(v1).all() and np.isfinite(p).all()
