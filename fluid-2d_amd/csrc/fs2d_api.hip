@@ -39,8 +39,8 @@ struct fs2d_handle {
     float* f[F_COUNT] = {};
     unsigned* max_bits = nullptr;
     unsigned* speed_slots = nullptr;  // kSpeedSlots * kSlotStride words, zero between uses
-    std::map<int, GsPlan> gs_plans;  // by sweep count
-    int gs_tw = 64;
+    std::map<std::pair<int, int>, GsPlan> gs_plans;  // by (sweep count, tile columns)
+    int gs_tw = 0;  // tile columns; 0: 128 for diffuse solves, 64 for pressure solves
     int gs_block = 8;
     int gs_lag = 4;
     int jacobi_kernel = FS2D_JACOBI_WAVE;
@@ -95,17 +95,26 @@ Scalars scalars(int R, float dt) {
     return s;
 }
 
-hipError_t gs_plan_for(fs2d_handle* h, int K, const GsPlan** out) {
-    auto it = h->gs_plans.find(K);
+// Tile columns of a Gauss-Seidel solve: the handle's setting, or by kind --
+// the diffuses' dense levels run faster with fewer, wider units, the pressure
+// solve's latency-bound levels with narrower ones (identical results).
+int gs_tile_cols_for(const fs2d_handle* h, int kind) {
+    if (h->gs_tw) return h->gs_tw;
+    return (kind == SWEEP_DIFFUSE && h->gs_block != 2) ? 128 : 64;
+}
+
+hipError_t gs_plan_for(fs2d_handle* h, int K, int kind, const GsPlan** out) {
+    const int tw = gs_tile_cols_for(h, kind);
+    auto it = h->gs_plans.find({K, tw});
     if (it == h->gs_plans.end()) {
         GsPlan plan{};
-        hipError_t e = (h->gs_block > 1) ? gs_unit_plan_create(&plan, h->R, K, h->gs_tw, h->gs_block, h->gs_lag)
-                                         : gs_plan_create(&plan, h->R, K, h->gs_tw);
+        hipError_t e = (h->gs_block > 1) ? gs_unit_plan_create(&plan, h->R, K, tw, h->gs_block, h->gs_lag)
+                                         : gs_plan_create(&plan, h->R, K, tw);
         if (e != hipSuccess) {
             gs_plan_destroy(&plan);
             return e;
         }
-        it = h->gs_plans.emplace(K, plan).first;
+        it = h->gs_plans.emplace(std::make_pair(K, tw), plan).first;
     }
     *out = &it->second;
     return hipSuccess;
@@ -177,7 +186,7 @@ int linear_solve(fs2d_handle* h, float*& x, const float* b, int kind, float a, f
     SweepArgs A = sweep_args(x, b, h->R, h->P, kind, a, denom, mode, K, zero_start);
     if (h->prm.sweep_order == FS2D_ORDER_GS_LEX) {
         const GsPlan* plan = nullptr;
-        FS2D_HIP(h, gs_plan_for(h, K, &plan));
+        FS2D_HIP(h, gs_plan_for(h, K, kind, &plan));
         if (plan->M > 0 && zero_start) {
             // the unit kernel starts from a zero-filled iterate, ring included
             // (zero_start == 2: the caller has filled it already)
@@ -264,7 +273,7 @@ int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
         set.a[1] = sweep_args(h->f[F_UN], h->f[F_X0U], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_H, Kd, 0);
         set.a[2] = sweep_args(h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_V, Kd, 0);
         const GsPlan* plan = nullptr;
-        FS2D_HIP(h, gs_plan_for(h, Kd, &plan));
+        FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
         FS2D_HIP(h, launch_gs_solve_set(h->stream, *plan, set));
     }
     if ((rc = advect_density(h, dt))) return rc;  // reads the pre-step u, v
@@ -455,7 +464,7 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     h->P = (rows + 63) / 64 * 64;
     h->field_bytes = sizeof(float) * static_cast<size_t>(h->R) * h->P;
     h->prm = prm;
-    h->gs_tw = prm.gs_tile_cols ? prm.gs_tile_cols : 64;
+    h->gs_tw = prm.gs_tile_cols;
     h->gs_block = prm.gs_block ? prm.gs_block : 8;
     h->gs_lag = prm.gs_lag ? prm.gs_lag : 4;
     h->jacobi_kernel = prm.jacobi_kernel ? prm.jacobi_kernel : FS2D_JACOBI_WAVE;
@@ -708,7 +717,7 @@ int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
         if (K <= 0) return 0;
         if (h->prm.sweep_order == FS2D_ORDER_JACOBI) return jacobi_launches(K, jacobi_depth_known(h, kind, K));
         const GsPlan* plan = nullptr;
-        if (gs_plan_for(h, K, &plan) != hipSuccess) return -1;
+        if (gs_plan_for(h, K, kind, &plan) != hipSuccess) return -1;
         return plan->levels;
     };
     const int d = solve_launches(Kd, SWEEP_DIFFUSE), p = solve_launches(Kp, SWEEP_PRESSURE);

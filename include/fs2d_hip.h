@@ -70,7 +70,9 @@ typedef struct fs2d_params {
     int pressure_iterations; /* K_p; the reference's literal is 20 (src/fluid_solver_cpu.cpp:25,30)    */
     int sweep_order;         /* enum fs2d_sweep_order                                                   */
     int jacobi_block;        /* Jacobi sweeps fused per launch (temporal block depth); 0 = autotuned    */
-    int gs_tile_cols;        /* Gauss-Seidel tile width in cells: 64 or 128 (128 takes gs_block 1, 4, 8); 0 = 64 */
+    int gs_tile_cols;        /* Gauss-Seidel tile width in cells: 64 or 128 (128 takes gs_block 1, 4, 8);
+                              * 0 = 128 for the diffuse solves and 64 for the pressure solves
+                              * (64 for both with gs_block 2); identical results            */
     int gs_block;            /* Gauss-Seidel sweeps per tile (temporal block): 1, 2, 4 or 8; 0 = default */
     int gs_lag;              /* steps each sweep trails the previous one in a tile: 1, 4 or 8; 0 = default */
     int jacobi_kernel;       /* enum fs2d_jacobi_kernel; 0 = default (register strips)                   */
