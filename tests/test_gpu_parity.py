@@ -257,6 +257,20 @@ def test_wide_tiles_vs_oracle(R, gsb, lag, oracle_lib):
         assert_same(got[f], want[f], f"wide step {name} R={R} gs_block={gsb} lag={lag}")
 
 
+@pytest.mark.parametrize("gsb,tw", [(8, 0), (8, 64), (8, 128), (4, 0), (2, 0)])
+def test_tile_width_by_solve_kind(gsb, tw):
+    """gs_tile_cols = 0 gives the diffuses 128-column units and the pressure
+    solves 64-column ones (64 for both with gs_block 2): the launches of a
+    step equal the level counts of the schedule model's unit plans."""
+    from schedule_models import gs_unit_plan
+    R, K = 258, 20
+    tw_d = tw or (64 if gsb == 2 else 128)
+    tw_p = tw or 64
+    d, p = len(gs_unit_plan(R, K, tw_d, gsb)), len(gs_unit_plan(R, K, tw_p, gsb))
+    with fs2d_amd.Solver(R, diffuse_iterations=K, pressure_iterations=K, gs_block=gsb, gs_tile_cols=tw) as s:
+        assert s.launches_per_step() == 3 + d + 1 + 2 * (2 + p) + 1, (d, p)
+
+
 # ------------------------------------------------ the reference's own outputs
 @pytest.mark.parametrize("R", [4, 6, 10, 18, 34])
 def test_golden_stages(R):
