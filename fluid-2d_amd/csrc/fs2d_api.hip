@@ -776,6 +776,16 @@ constexpr int kMgPre = 2, kMgPost = 2, kMgMaxCycles = 30;
 // per sweep, so n^2 sweeps reduce it by ~50x (64 at least, 4096 at most).
 static int mg_coarsest_sweeps(int n) { return std::min(4096, std::max(64, n * n)); }
 
+static void mg_free(fs2d_handle* h) {
+    for (auto& L : h->mg)
+        for (float* q : {L.p, L.t, L.b})
+            if (q) (void)hipFree(q);
+    h->mg.clear();
+}
+
+// Coarse grids of the multigrid seed.  Either every level is allocated and
+// cleared, or none is kept: a level enters h->mg only with all three of its
+// arrays, and any failure frees what was allocated so far.
 static int mg_levels(fs2d_handle* h) {
     if (!h->mg.empty()) return FS2D_OK;
     int n = h->R - 2;
@@ -783,12 +793,17 @@ static int mg_levels(fs2d_handle* h) {
         n /= 2;
         fs2d_handle::MgLevel L{n, n + 2, (n + 2 + 63) / 64 * 64, nullptr, nullptr, nullptr};
         const size_t bytes = sizeof(float) * static_cast<size_t>(L.R) * L.P;
+        hipError_t e = hipMalloc(&L.p, bytes);
+        if (e == hipSuccess) e = hipMalloc(&L.t, bytes);
+        if (e == hipSuccess) e = hipMalloc(&L.b, bytes);
+        if (e == hipSuccess) e = launch_fill(h->stream, L.b, 0u, bytes / 4);
+        if (e != hipSuccess) {
+            for (float* q : {L.p, L.t, L.b})
+                if (q) (void)hipFree(q);
+            mg_free(h);
+            return fail(h, FS2D_ERR_HIP, std::string("multigrid levels: ") + hipGetErrorString(e));
+        }
         h->mg.push_back(L);  // owned by the handle from here on, freed in destroy
-        fs2d_handle::MgLevel& M = h->mg.back();
-        FS2D_HIP(h, hipMalloc(&M.p, bytes));
-        FS2D_HIP(h, hipMalloc(&M.t, bytes));
-        FS2D_HIP(h, hipMalloc(&M.b, bytes));
-        FS2D_HIP(h, launch_fill(h->stream, M.b, 0u, bytes / 4));
     }
     return FS2D_OK;
 }
@@ -843,6 +858,8 @@ int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, 
     int fine_smooth = 0;  // weighted Jacobi sweeps of the fine grid in V-cycles
     if (seed == 3) {
         if ((rc = mg_levels(h))) return rc;
+        for (const auto& L : h->mg)
+            if (!L.p || !L.t || !L.b) return fail(h, FS2D_ERR_STATE, "fs2d_hip_project_seeded: multigrid level not allocated");
         for (int c = 0; c < kMgMaxCycles; ++c) {
             if ((rc = mg_vcycle(h, 0))) return rc;
             fine_smooth += h->mg.empty() ? mg_coarsest_sweeps(h->R - 2) : kMgPre + kMgPost;

@@ -418,7 +418,7 @@ def big_state():
 
 
 @pytest.mark.parametrize("order,gsb,lag,block,K", [(GS, 1, 0, 0, 3), (GS, 4, 1, 0, 3), (GS, 8, 8, 0, 3),
-                                                   (JAC, 0, 0, 4, 8), (JAC, 0, 0, 8, 16)])
+                                                   (GS, 8, 4, 0, 20), (JAC, 0, 0, 4, 8), (JAC, 0, 0, 8, 16)])
 def test_full_size_stages(big_state, order, gsb, lag, block, K, oracle_lib):
     """4096^2 interior: each stage against the oracle with short solves (the
     oracle cannot afford 80 sweeps here); exercises the full tile grid."""
