@@ -65,6 +65,8 @@ struct fs2d_handle {
     float*& v() { return f[F_V]; }
 };
 
+void fs2d::set_handle_free_error(const std::string& msg) { g_create_error = msg; }
+
 namespace {
 
 int fail(fs2d_handle* h, int code, const std::string& msg) {

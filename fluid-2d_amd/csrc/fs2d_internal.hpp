@@ -76,6 +76,7 @@ struct SweepSet {
     SweepArgs a[kMax];
     int count;
     int units_per_field;  // set per launch
+    int level, gi, gj, gk;  // launches without a work list: the level and the unit grid (GI, GJ, GK)
 };
 
 // ---- launchers (stream-ordered, no synchronisation) ---------------------
@@ -141,6 +142,11 @@ hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW);
 hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int lag);
 void gs_plan_destroy(GsPlan* plan);
 hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a);
+// The same solve with the lagged unit kernel (M sweeps per unit, lag steps)
+// but no plan: each level's units are enumerated by the kernel from its block
+// index, so nothing is allocated or copied (K <= kImplicitMaxSweeps).
+constexpr int kImplicitMaxSweeps = 4096;
+hipError_t launch_gs_solve_implicit(hipStream_t s, int R, int K, int TW, int M, int lag, const SweepArgs& a);
 // All solves of the set in shared level launches (unit plans; a tile plan takes one).
 hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set);
 
@@ -155,6 +161,12 @@ hipError_t launch_jacobi_pass(hipStream_t s, const SweepArgs& a, const float* sr
                               int kernel);
 int jacobi_launches(int K, int block);
 
+}  // namespace fs2d
+
+#include <string>
+namespace fs2d {
+// The message fs2d_hip_last_error(NULL) returns (calls that take no handle).
+void set_handle_free_error(const std::string& msg);
 }  // namespace fs2d
 
 struct fs2d_net;

@@ -233,6 +233,23 @@ __global__ __launch_bounds__(64) void k_gs_tile(SweepArgs A, const int2* __restr
 // overwritten at step t + 2, so the M waves relax in place in LDS, each
 // trailing the previous sweep's wave (the lagged schedule below).  Units with
 // equal I' + J' + kappa touch disjoint cells and are launched together.
+// Unit (I', J', kappa) of block b of a level launch: from the plan's work
+// list, or, with no list, the b-th unit of level S.level in (kappa, I') order
+// over the whole unit grid.  The implicit order includes units that hold no
+// interior cell; such a unit relaxes and writes nothing.
+__device__ __forceinline__ int2 unit_of(const SweepSet& S, const int2* __restrict__ work, int b) {
+    if (work) return work[b];
+    const int L = S.level;
+    const int kap_lo = max(0, L - (S.gi - 1) - (S.gj - 1)), kap_hi = min(S.gk - 1, L);
+    for (int kap = kap_lo; kap <= kap_hi; ++kap) {
+        const int m = L - kap;
+        const int lo = max(0, m - (S.gj - 1)), hi = min(S.gi - 1, m);
+        if (b <= hi - lo) return make_int2(((lo + b) << 16) | (m - lo - b), kap);
+        b -= hi - lo + 1;
+    }
+    return make_int2(0, 0);  // not reached: a level launches exactly its unit count
+}
+
 template <int TW, int M, int L, int KIND>
 __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __restrict__ work) {
     // blocks [f * units, (f + 1) * units) relax field f of the set, unit by unit
@@ -251,7 +268,7 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
     __shared__ float X[XH * XP + GUARD];
     __shared__ float Bs[BH * BP + GUARD];
 
-    const int2 wk = work[blockIdx.x - f * S.units_per_field];
+    const int2 wk = unit_of(S, work, blockIdx.x - f * S.units_per_field);
     const int I = wk.x >> 16, J = wk.x & 0xffff, kap = wk.y;
     const int R = A.R, P = A.P, n = R - 2, K = A.K;
     const int k_lo = kap * M, k_hi = min(k_lo + M, K) - 1, k_top = k_lo + M - 1;
@@ -830,6 +847,50 @@ hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet
             }
             if (!ok) return hipErrorInvalidValue;
         }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Units of level L of the implicit enumeration (unit_of without a list).
+static int implicit_level_units(int GI, int GJ, int GK, int L) {
+    int count = 0;
+    for (int kap = std::max(0, L - (GI - 1) - (GJ - 1)); kap <= std::min(GK - 1, L); ++kap) {
+        const int m = L - kap;
+        count += std::min(GI - 1, m) - std::max(0, m - (GJ - 1)) + 1;
+    }
+    return count;
+}
+
+hipError_t launch_gs_solve_implicit(hipStream_t s, int R, int K, int TW, int M, int lag, const SweepArgs& a) {
+    if (K <= 0) return hipSuccess;
+    if (K > kImplicitMaxSweeps || (TW != 64 && TW != 128) || (M != 2 && M != 4 && M != 8) ||
+        (TW == 128 && M == 2))
+        return hipErrorInvalidValue;
+    const int n = R - 2;
+    SweepSet set{};
+    set.a[0] = a;
+    set.count = 1;
+    set.gi = (n + K - 1 + GS_ROWS - 1) / GS_ROWS;
+    set.gj = (n + K - 1 + TW - 1) / TW;
+    set.gk = (K + M - 1) / M;
+    for (int L = 0; L <= set.gi + set.gj + set.gk - 3; ++L) {
+        const int units = implicit_level_units(set.gi, set.gj, set.gk, L);
+        if (units <= 0) continue;
+        set.level = L;
+        bool ok = false;
+        if (TW == 64) {
+            switch (M) {
+                case 2: ok = launch_unit_lag<64, 2>(s, lag, units, nullptr, set); break;
+                case 4: ok = launch_unit_lag<64, 4>(s, lag, units, nullptr, set); break;
+                default: ok = launch_unit_lag<64, 8>(s, lag, units, nullptr, set); break;
+            }
+        } else {
+            ok = (M == 4) ? launch_unit_lag<128, 4>(s, lag, units, nullptr, set)
+                          : launch_unit_lag<128, 8>(s, lag, units, nullptr, set);
+        }
+        if (!ok) return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -98,6 +98,7 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_launches_per_step": (_i, [H, ctypes.POINTER(_i)]),
         "fs2d_hip_last_error": (ctypes.c_char_p, [H]),
         "fs2d_hip_abi_version": (_i, []),
+        "fs2d_hip_project_noalloc": (_i, [_i, ctypes.POINTER(Params), _p, _p, _p, _p]),
         "fs2d_hip_render_density": (_i, [H, ctypes.POINTER(ColorMultipliers), _p]),
         "fs2d_hip_render_density_device": (_i, [H, ctypes.POINTER(ColorMultipliers), _p]),
         "fs2d_hip_render_velocity": (_i, [H, _f, _f, _p]),
@@ -278,8 +279,29 @@ EXPORTED_SYMBOLS = (
     "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_launches_per_step",
     "fs2d_hip_last_error", "fs2d_hip_abi_version",
     "fs2d_hip_render_density", "fs2d_hip_render_density_device", "fs2d_hip_render_velocity",
-    "fs2d_hip_render_velocity_device", "fs2d_hip_time_render",
+    "fs2d_hip_render_velocity_device", "fs2d_hip_time_render", "fs2d_hip_project_noalloc",
 )
+
+
+def project_noalloc(rows: int, u_ptr: int, v_ptr: int, pressure_ptr: int | None = None, stream: int | None = None,
+                    pressure_iterations: int = 20, order: int = GS_LEX, jacobi_block: int = 0, gs_block: int = 0,
+                    gs_lag: int = 0, gs_tile_cols: int = 0, jacobi_kernel: int = 0) -> None:
+    """fs2d_hip_project_noalloc: the projection on caller-owned device arrays
+    (addresses of rows x rows float32, row-major) enqueued on `stream` (a
+    hipStream_t handle; None = the null stream); allocates nothing."""
+    L = library()
+    prm = Params()
+    L.fs2d_default_params(ctypes.byref(prm))
+    prm.pressure_iterations = pressure_iterations
+    prm.sweep_order = order
+    prm.jacobi_block = jacobi_block
+    prm.gs_block = gs_block
+    prm.gs_lag = gs_lag
+    prm.gs_tile_cols = gs_tile_cols
+    prm.jacobi_kernel = jacobi_kernel
+    rc = L.fs2d_hip_project_noalloc(rows, ctypes.byref(prm), u_ptr, v_ptr, pressure_ptr, stream)
+    if rc != 0:
+        raise Fs2dError(f"fs2d_hip_project_noalloc failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
 
 
 class Batch:

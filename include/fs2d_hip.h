@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FS2D_ABI_VERSION 2
+#define FS2D_ABI_VERSION 3
 
 enum fs2d_status {
     FS2D_OK = 0,
@@ -155,7 +155,30 @@ int fs2d_hip_copy_bandwidth(long long bytes, int reps, float* gb_per_s);
 /* Number of kernel launches one step enqueues (for per-launch averages). */
 int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches);
 
-/* Last error of the handle (or of the last failed create when h is NULL). */
+/* ---- projection without allocation -------------------------------------
+ * No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The projection of fs2d_hip_stage_project
+ * (src/fluid_solver_cpu.cpp:179-215: divergence, params->pressure_iterations
+ * sweeps from a zero pressure, gradient subtraction, opposite rings) on
+ * caller-owned device arrays u and v (rows x rows floats each, row-major,
+ * ring included, updated in place), enqueued on `stream` (a hipStream_t; NULL
+ * = the null stream), returning without waiting.  pressure (may be NULL)
+ * receives the final pressure, rows x rows floats.  Results equal
+ * fs2d_hip_stage_project's bit for bit.
+ *
+ * Allocates nothing: its scratch (divergence, pressure, the pressure's Jacobi
+ * partner) is one file-scope device array sized for rows <= 1026 (12.6 MB per
+ * GPU, held until the process ends); larger grids are rejected.  Calls that
+ * overlap on one GPU share that array and race: serialise them.  Orders: the
+ * Jacobi order (jacobi_block 0 = 4 sweeps per launch, not tuned) or
+ * Gauss-Seidel in units of gs_block 2, 4 or 8 sweeps (pressure_iterations <=
+ * 4096).  Errors: fs2d_hip_last_error(NULL). */
+int fs2d_hip_project_noalloc(int rows, const fs2d_params* params, float* u, float* v, float* pressure,
+                             void* stream);
+
+/* Last error of the handle (or, when h is NULL, of the last failed create or
+ * handle-free call). */
 const char* fs2d_hip_last_error(const fs2d_handle* h);
 int fs2d_hip_abi_version(void);
 

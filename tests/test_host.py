@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, missing
     assert set(syms) == set(fs2d_amd.EXPORTED_SYMBOLS)
-    assert lib.fs2d_hip_abi_version() == 2
+    assert lib.fs2d_hip_abi_version() == 3
     assert ctypes.sizeof(fs2d_amd.Params) == 9 * 4
 
 
