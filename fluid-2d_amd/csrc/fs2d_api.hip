@@ -49,6 +49,7 @@ struct fs2d_handle {
     bool speed_valid = false;   // max_bits holds max |u|, |v| of the current velocity
     long long steps_done = 0;
     double* res_sums = nullptr;  // 2 doubles: residual sums of the seeded projection
+    void* sort_buf = nullptr;    // the density scatter's sorted path (scatter_sort_bytes), on the first advect
     void* render_buf = nullptr;  // device staging of the visual output, grown on demand
     size_t render_bytes = 0;
     struct MgLevel {
@@ -240,8 +241,17 @@ int advect_density(fs2d_handle* h, float dt) {
         FS2D_HIP(h, launch_max_speed(h->stream, h->max_bits, h->u(), h->v(), h->R, h->P));
         h->speed_valid = true;
     }
+    if (!h->sort_buf) {
+        // A handle's first advect runs eagerly (the first step is never
+        // captured), so this allocation never happens inside a capture.
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        FS2D_HIP(h, hipStreamIsCapturing(h->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(h, FS2D_ERR_STATE, "advect_density: the scatter's buffers must exist before a capture");
+        FS2D_HIP(h, hipMalloc(&h->sort_buf, scatter_sort_bytes(h->R)));
+    }
     FS2D_HIP(h, launch_scatter_density(h->stream, h->f[F_T1], h->rho(), h->u(), h->v(), h->R, h->P, s.dt0,
-                                       h->max_bits));
+                                       h->max_bits, h->sort_buf));
     std::swap(h->rho(), h->f[F_T1]);
     return FS2D_OK;
 }
@@ -502,6 +512,7 @@ int fs2d_hip_destroy(fs2d_handle* h) {
     drop_graphs(h);
     if (h->res_sums) (void)hipFree(h->res_sums);
     if (h->render_buf) (void)hipFree(h->render_buf);
+    if (h->sort_buf) (void)hipFree(h->sort_buf);
     for (auto& L : h->mg)
         for (float* q : {L.p, L.t, L.b})
             if (q) (void)hipFree(q);
@@ -748,14 +759,16 @@ int fs2d_hip_pressure_residual(fs2d_handle* h, double* rel) {
 }
 
 // Jacobi launches on the handle's pressure field until the relative residual
-// is <= tol (checked every `check` sweeps) or max_sweeps have run.
+// is <= tol (checked every `check` sweeps) or max_sweeps have run.  A
+// non-finite residual stops it at once: a NaN or infinite iterate stays
+// non-finite under the sweeps, so the budget would be spent for nothing.
 static int jacobi_until(fs2d_handle* h, float tol, int max_sweeps, int check, int* sweeps, double* rel) {
     const int depth = jacobi_depth_known(h, SWEEP_PRESSURE, h->prm.pressure_iterations);
     const SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, max_sweeps, 0);
     int rc;
     if ((rc = residual_of(h, rel))) return rc;
     int done = 0, since = 0;
-    while (done < max_sweeps && !(*rel <= tol)) {
+    while (done < max_sweeps && !(*rel <= tol) && std::isfinite(*rel)) {
         const int m = std::min(depth, max_sweeps - done);
         FS2D_HIP(h, launch_jacobi_pass(h->stream, A, h->f[F_P], h->f[F_SCRATCH], 0, m, h->jacobi_kernel));
         std::swap(h->f[F_P], h->f[F_SCRATCH]);

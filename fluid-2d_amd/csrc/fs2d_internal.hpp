@@ -86,8 +86,13 @@ hipError_t launch_add_sources(hipStream_t s, float* x, float* x0, const float* s
 hipError_t launch_add_sources_split(hipStream_t s, float* out, float* x0, const float* x, const float* src, int R,
                                     int P, float dt);
 hipError_t launch_max_speed(hipStream_t s, unsigned* out_bits, const float* u, const float* v, int R, int P);
-hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, const float* u,
-                                  const float* v, int R, int P, float dt0, const unsigned* max_bits);
+// Density advection (fs2d_scatter.hip) into out (interior and ring); max_bits
+// holds max |u|, |v|.  sort_scratch (scatter_sort_bytes(R) bytes, may be null
+// when every source is known to stay within the window path's radius) serves
+// the sorted path taken when some source moves farther.
+hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, const float* u, const float* v, int R,
+                                  int P, float dt0, const unsigned* max_bits, void* sort_scratch);
+size_t scatter_sort_bytes(int R);
 hipError_t launch_gather_velocity(hipStream_t s, float* u1, float* v1, const float* u0, const float* v0,
                                   int R, int P, float dt0);
 // p_zero (may be null): also zero this pressure iterate, ring included.

@@ -38,7 +38,8 @@ __global__ __launch_bounds__(BX* BY) void k_add_sources_split(float* __restrict_
 }
 
 // Largest finite |u|, |v| over the interior, as float bits (non-negative
-// floats order like their bit patterns).  Bounds the scatter's search window.
+// floats order like their bit patterns).  Bounds how far a source moves in the
+// density scatter.
 __global__ __launch_bounds__(256) void k_max_speed(unsigned* __restrict__ out, const float* __restrict__ u,
                                                    const float* __restrict__ v, int R, int P) {
     float m = 0.f;
@@ -49,13 +50,6 @@ __global__ __launch_bounds__(256) void k_max_speed(unsigned* __restrict__ out, c
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
-}
-
-__global__ __launch_bounds__(BX* BY) void k_scatter_density(float* __restrict__ out, const float* __restrict__ rho0,
-                                                            const float* __restrict__ u, const float* __restrict__ v,
-                                                            int R, int P, float dt0, const unsigned* __restrict__ max_bits) {
-    int di, dj;
-    if (interior(R, di, dj)) cells::scatter_density(out, rho0, u, v, R, P, dt0, __uint_as_float(*max_bits), di, dj);
 }
 
 __global__ __launch_bounds__(BX* BY) void k_gather_velocity(float* __restrict__ u1, float* __restrict__ v1,
@@ -178,12 +172,6 @@ hipError_t launch_add_sources_split(hipStream_t s, float* out, float* x0, const 
 
 hipError_t launch_max_speed(hipStream_t s, unsigned* out_bits, const float* u, const float* v, int R, int P) {
     hipLaunchKernelGGL(k_max_speed, dim3(std::min(R - 2, 2048)), dim3(256), 0, s, out_bits, u, v, R, P);
-    return hipGetLastError();
-}
-
-hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, const float* u, const float* v,
-                                  int R, int P, float dt0, const unsigned* max_bits) {
-    hipLaunchKernelGGL(k_scatter_density, grid2d(R), dim3(BX, BY), 0, s, out, rho0, u, v, R, P, dt0, max_bits);
     return hipGetLastError();
 }
 

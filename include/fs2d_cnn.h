@@ -79,10 +79,12 @@ const char* fs2d_net_last_error(const fs2d_net* net);
  *   ignored).
  * Jacobi sweeps run in launches of the handle's depth until the compact
  * Poisson residual ||d - L p||_2 <= tol * ||d||_2 (checked every `check`
- * sweeps, rounded up to whole launches) or max_sweeps is reached; then the
- * gradient is applied.  With seed = 2 and fallback != 0, a seeded solve that
- * misses tol within max_sweeps restarts from the previous pressure (seed 1)
- * with the same budget.  Reports the sweeps run, the final relative residual
+ * sweeps, rounded up to whole launches), max_sweeps is reached or the
+ * residual is not finite (a NaN or infinite iterate stays so under the
+ * sweeps); then the gradient is applied.  With seed = 2 and fallback != 0, a
+ * seeded solve that misses tol -- a non-finite network guess at once, without
+ * sweeps -- restarts from the previous pressure (seed 1) with the same
+ * budget.  Reports the sweeps run, the final relative residual
  * and whether the fallback ran (sweeps_run counts the fine grid's sweeps,
  * V-cycle smoothing included).  Requires the Jacobi order and a grid side
  * n = rows - 2 that is a multiple of 32 for seed 2. */

@@ -197,6 +197,75 @@ def test_seeded_projection_falls_back():
     assert fb and sw == 16 and rel > 1e-12
 
 
+def projected(R, st, seed, net, tol, max_sweeps, check, steps=2):
+    """Seeded projection from `st` after `steps` plain steps: (sweeps, rel,
+    fell back, u, v, pressure, u and v before the projection)."""
+    with fs2d_amd.Solver(R, diffuse_iterations=20, pressure_iterations=40, order=fs2d_amd.JACOBI) as s:
+        s.upload(*st)
+        s.set_sources(None, None, None)
+        s.step_n(steps, 1e-5, 1e-6, 0.05)
+        _, u0, v0 = s.download()
+        sw, rel, fb, _ = PN.seeded_projection(s, net, seed, tol, max_sweeps, check=check)
+        _, u1, v1 = s.download()
+        p = s.pressure()
+    return sw, rel, fb, u1, v1, p, u0, v0
+
+
+def test_nan_network_falls_back_at_once():
+    """A network whose weights are all NaN predicts NaN: the seeded solve stops
+    before its first sweep and the fallback from the previous pressure runs --
+    the same sweeps and bit-identical fields as seed 1 from the same state."""
+    R = 66
+    st = W.vortex_state(R, seed=3, n_vortices=16)
+    with PN.PressureNet(seed=1) as net:
+        net.set_params(np.full(net.param_count, np.nan, np.float32))
+        sw, rel, fb, u, v, p, _, _ = projected(R, st, 2, net, 1e-4, 200000, 256)
+    sw1, rel1, fb1, u_1, v_1, p_1, _, _ = projected(R, st, 1, None, 1e-4, 200000, 256)
+    assert fb and not fb1 and rel <= 1e-4 and sw == sw1 and rel == rel1, (sw, sw1, rel, rel1)
+    assert np.isfinite(p).all() and np.array_equal(p, p_1)
+    assert np.array_equal(u, u_1) and np.array_equal(v, v_1)
+
+
+def test_zero_weight_network_still_reaches_the_target():
+    """All-zero weights make the network's correction zero, so its guess is the
+    previous pressure (up to the rounding of its input scaling): the solve
+    reaches the target with or without the fallback, and the projection
+    agrees with seed 1's."""
+    R = 66
+    st = W.vortex_state(R, seed=5, n_vortices=16)
+    with PN.PressureNet(seed=1) as net:
+        net.set_params(np.zeros(net.param_count, np.float32))
+        sw, rel, fb, u, v, _, u0, v0 = projected(R, st, 2, net, 1e-4, 200000, 256)
+    sw1, rel1, _, u_1, v_1, _, _, _ = projected(R, st, 1, None, 1e-4, 200000, 256)
+    assert rel <= 1e-4 and rel1 <= 1e-4 and np.isfinite(u).all() and np.isfinite(v).all()
+    change = np.linalg.norm(np.concatenate([(u_1 - u0).ravel(), (v_1 - v0).ravel()]))
+    diff = np.linalg.norm(np.concatenate([(u - u_1).ravel(), (v - v_1).ravel()]))
+    assert change > 0 and diff <= 0.05 * change, (diff, change, sw, sw1, fb)
+
+
+def test_network_seed_at_1024():
+    """The network seed with the fallback on the 1024^2 interior (64 vortices),
+    to a residual above float32's floor there (the exact pressure rounded to
+    float32 leaves ~1.6e-3): it reaches the target, and the residual it reports
+    is the compact residual of the pressure it leaves, recomputed here in
+    float64 from the divergence of the velocity before the projection."""
+    R = 1026
+    tol = 0.5
+    st = W.vortex_state(R, seed=8)
+    with PN.PressureNet(seed=4) as net:
+        sw, rel, fb, u, v, p, u0, v0 = projected(R, st, 2, net, tol, 65536, 1024)
+    assert rel <= tol, (sw, rel, fb)
+    assert np.isfinite(u).all() and np.isfinite(v).all() and np.isfinite(p).all()
+    d = PN.divergence(u0, v0)
+    q = p.astype(np.float64)
+    Lp = 4 * q[1:-1, 1:-1] - (q[:-2, 1:-1] + q[2:, 1:-1] + q[1:-1, :-2] + q[1:-1, 2:])
+    want = np.linalg.norm(d - Lp) / np.linalg.norm(d)
+    assert abs(rel - want) <= 1e-3 * want, (rel, want)
+    sw1, rel1, _, _, _, _, _, _ = projected(R, st, 1, None, tol, 65536, 1024)
+    assert rel1 <= tol
+    print(f"R=1026 tol {tol}: network seed {sw} sweeps (fallback {fb}), previous pressure {sw1} sweeps")
+
+
 def test_forward_timing_and_flop_count():
     with PN.PressureNet(seed=0) as net:
         ms = net.time_forward(1, 64, 64, reps=3)

@@ -115,6 +115,52 @@ def test_scatter_density(R, vel, oracle_lib):
     assert_same(got, want, f"scatter R={R} vel={vel}")
 
 
+@pytest.mark.parametrize("fast", [(5.0, 0.0), (-3.0, 2.5), (1e6, 0.0)])
+def test_scatter_one_fast_cell(fast, oracle_lib):
+    """A slow field with one fast cell at 1024^2: the largest speed, not the
+    typical one, sets how far sources may move (513 cells for u = 5; a source
+    at 1e6 leaves the grid).  The sorted path's work does not depend on it:
+    bit for bit against the oracle, and the advect finishes in bounded time."""
+    import time
+    R = 1026
+    rho, u, v, *_ = W.random_state(R, seed=77, vel_scale=0.002)
+    u, v = u.copy(), v.copy()
+    u[400, 300], v[400, 300] = fast
+    st = (rho, u, v, None, None, None)
+    with fs2d_amd.Solver(R) as s:
+        s.upload(rho, u, v)
+        s.advect_density(0.1)
+        s.sync()
+        s.upload(rho, u, v)
+        t0 = time.perf_counter()
+        s.advect_density(0.1)
+        s.sync()
+        elapsed = time.perf_counter() - t0
+        got = s.download()[0]
+    want = rho.copy()
+    oracle_lib.advect(want, u, v, 0, 0.1, True)
+    assert_same(got, want, f"scatter with one fast cell {fast}")
+    assert elapsed < 0.05, f"advect took {elapsed * 1e3:.1f} ms"
+
+
+@pytest.mark.parametrize("reach", [0.5, 1.5, 2.9, 3.9, 4.1, 5.5, 9.0])
+def test_scatter_around_the_window_limit(reach, oracle_lib):
+    """Every radius of the window path (1 to 4: sources reach at most `reach`
+    cells) and the sorted path just past it."""
+    R = 300
+    rho, u, v, *_ = W.random_state(R, seed=int(reach * 10), vel_scale=1.0)
+    dt = 0.1
+    scale = np.float32(reach / (np.sqrt(np.float32(R * R)) * dt) / max(np.abs(u).max(), np.abs(v).max()))
+    u, v = (u * scale).astype(np.float32), (v * scale).astype(np.float32)
+    with fs2d_amd.Solver(R) as s:
+        s.upload(rho, u, v)
+        s.advect_density(dt)
+        got = s.download()[0]
+    want = rho.copy()
+    oracle_lib.advect(want, u, v, 0, dt, True)
+    assert_same(got, want, f"scatter reach={reach}")
+
+
 @pytest.mark.parametrize("R", SIZES)
 @pytest.mark.parametrize("vel", [0.0, 0.05, 1.0, 6.0])
 def test_gather_velocity(R, vel, oracle_lib):
@@ -210,6 +256,27 @@ def test_project(R, order, block, gsb, lag, jk, oracle_lib):
         assert_same(gp, wp, f"pressure R={R} K={K} order={order} gs_block={gsb} lag={lag} block={block} jk={jk}")
         assert_same(gu, wu, f"project u R={R} K={K} order={order}")
         assert_same(gv, wv, f"project v R={R} K={K} order={order}")
+
+
+@pytest.mark.parametrize("order", [GS, JAC])
+def test_projection_against_float64_oracle(order, oracle_lib):
+    """The float32 projection against the same projection carried out in
+    float64 (the oracle's fso_project_f64, same order and sweep count): the
+    velocities agree to float32 rounding, and so does the post-projection
+    divergence, computed in float64 from each result."""
+    R, K = 258, 40
+    rho, u, v = W.vortex_state(R, seed=23, n_vortices=16)
+    with make(R, (rho, u, v, None, None, None), order=order, kp=K) as s:
+        s.project(K)
+        _, gu, gv = s.download()
+    ud, vd = u.astype(np.float64), v.astype(np.float64)
+    oracle_lib.project_f64(ud, vd, K, oracle_order(order))
+    scale = max(np.abs(ud).max(), np.abs(vd).max())
+    assert np.abs(gu - ud).max() < 1e-5 * scale and np.abs(gv - vd).max() < 1e-5 * scale
+    n32 = oracle_lib.divergence_norm_f64(gu, gv)
+    n64 = oracle_lib.divergence_norm_f64(ud.astype(np.float32), vd.astype(np.float32))
+    n0 = oracle_lib.divergence_norm_f64(u, v)
+    assert n64 < n0 and abs(n32 - n64) <= 1e-3 * n64, (n32, n64, n0)
 
 
 # 64 x 128 tiles: units of 128 columns (and the one-sweep tile kernel at 128 columns)

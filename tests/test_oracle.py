@@ -331,3 +331,65 @@ def test_render_velocity_length_and_direction(oracle_lib):
     dx = L["end"]["x"][::8, ::8].astype(np.float64) - L["start"]["x"][::8, ::8]
     want = 250000.0 * u[::8, ::8].astype(np.float64) / s
     assert np.allclose(dx, want, rtol=1e-5, atol=2e-4)
+
+
+def test_jacobi_projection_is_matrix_iteration(oracle_lib):
+    """The Jacobi-order projection: p_{k+1} = (d + N p_k) / 4 on the interior
+    from p_0 = 0 with the continuous ring after every sweep (N sums the four
+    neighbours), then u -= 0.5 (pE - pW) / h, v -= 0.5 (pS - pN) / h with the
+    opposite rings -- replayed in float64 from the oracle's own divergence.  A
+    Gauss-Seidel/Jacobi mix-up, a dropped ring or a wrong sign fails it."""
+    R = 13
+    _, u, v, *_ = W.random_state(R, seed=21, vel_scale=1.0)
+    K = 4
+    a, b = u.copy(), v.copy()
+    p = oracle_lib.project(a, b, K, JAC, want_p=True)
+    d = oracle_lib.divergence(u, v).astype(np.float64)
+    q = np.zeros((R, R))
+    for _ in range(K):
+        nb = q[:-2, 1:-1] + q[2:, 1:-1] + q[1:-1, :-2] + q[1:-1, 2:]
+        new = q.copy()
+        new[1:-1, 1:-1] = (d[1:-1, 1:-1] + nb) / 4.0
+        W.ring(new, 0)
+        q = new
+    assert np.abs(p - q).max() < 1e-6 * np.abs(q).max()
+    h = 1.0 / R
+    uu, vv = u.astype(np.float64), v.astype(np.float64)
+    uu[1:-1, 1:-1] -= 0.5 * (q[1:-1, 2:] - q[1:-1, :-2]) / h
+    vv[1:-1, 1:-1] -= 0.5 * (q[2:, 1:-1] - q[:-2, 1:-1]) / h
+    W.ring(uu, 1)
+    W.ring(vv, 2)
+    assert np.abs(a - uu).max() < 1e-5 * np.abs(uu).max()
+    assert np.abs(b - vv).max() < 1e-5 * np.abs(vv).max()
+    pg = oracle_lib.project(u.copy(), v.copy(), K, GS, want_p=True)
+    assert np.abs(pg - q).max() > 1e-3 * np.abs(q).max()  # the Gauss-Seidel order differs
+
+
+def test_poisson_residual_closed_forms(oracle_lib):
+    """The compact residual ||d - A p|| / ||d|| (A p = 4p - sum of neighbours,
+    continuous ring): exactly 1 for p = 0 and for any constant p (the
+    operator's null space); below 1e-5 for the exact solution (DCT, float64,
+    rounded to float32); and a hand-worked 2 x 2 interior value."""
+    from fs2d_amd import pressure_net
+
+    R = 34
+    _, u, v = W.vortex_state(R, seed=22, n_vortices=8)  # opposite rings: sum(d) telescopes to ~0
+    d = oracle_lib.divergence(u, v)
+    assert oracle_lib.poisson_residual_f64(d, np.zeros((R, R), np.float32)) == 1.0
+    assert oracle_lib.poisson_residual_f64(d, np.full((R, R), 3.25, np.float32)) == 1.0
+    exact = pressure_net.poisson_dct(d.astype(np.float64)[1:-1, 1:-1])
+    p = np.zeros((R, R), np.float32)
+    p[1:-1, 1:-1] = exact.astype(np.float32)
+    W.ring(p, 0)
+    assert oracle_lib.poisson_residual_f64(d, p) < 1e-5
+    # R = 4: interior cells (1,1) (1,2) (2,1) (2,2); with the continuous ring an
+    # interior neighbour outside the 2 x 2 block is the cell itself.
+    d4 = np.zeros((4, 4), np.float32)
+    d4[1:3, 1:3] = [[1.0, 2.0], [3.0, 4.0]]
+    p4 = np.zeros((4, 4), np.float32)
+    p4[1:3, 1:3] = [[1.0, 0.0], [0.0, 0.0]]
+    W.ring(p4, 0)
+    # A p at (1,1): 4*1 - (1 + 0 + 1 + 0) = 2; at (1,2) and (2,1): 0 - 1 = -1; at (2,2): 0
+    r = np.array([1.0 - 2.0, 2.0 + 1.0, 3.0 + 1.0, 4.0])
+    want = np.sqrt((r ** 2).sum() / 30.0)
+    assert abs(oracle_lib.poisson_residual_f64(d4, p4) - want) < 1e-15
