@@ -35,18 +35,22 @@ for p in (ROOT, os.path.join(ROOT, "fluid-2d_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_unit_pressure_4096_traffic.txt")
+# wave64 VALU instructions per second the chip can issue: 256 CUs x 4 SIMDs,
+# one wave64 instruction per 4 cycles per SIMD (16 lanes), 2.4 GHz
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
-def committed_traffic(order: int, n: int, K: int):
-    """HBM bytes of one pressure solve measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE (calibrated against a plain copy) for the Gauss-Seidel
-    4096^2, K=80 configuration; None for any other configuration."""
-    if order != 0 or n != 4096 or K != 80 or not os.path.exists(TRAFFIC_FILE):
+def profile_value(order: int, n: int, K: int, key: str):
+    """A figure from the committed rocprofv3 summary of this configuration
+    (profiles/{gs,jacobi}_solves_{n}_traffic.txt, K = 80): HBM bytes per solve
+    from FETCH_SIZE / WRITE_SIZE calibrated against a plain copy, or VALU
+    instructions per launch; None where no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"{'gs' if order == 0 else 'jacobi'}_solves_{n}_traffic.txt")
+    if K != 80 or not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic_bytes_per_pressure_solve:"):
-            return int(float(line.split(":", 1)[1]))
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1])
     return None
 
 
@@ -88,13 +92,46 @@ def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state
     s.step_n(steps, diff, visc, dt)
     s.sync()
     elapsed = time.perf_counter() - t0
-    # dominant kernel: the pressure relaxation (K_p sweeps), HIP events on the
-    # solver's own stream around repeated solves
-    reps = max(3, min(20, int(0.5 / max(elapsed / steps / 5, 1e-4))))
-    ms_solve = s.time_pressure_sweeps(reps) if time_solve else None
-    launches = s.launches_per_step()
+    # the step's two relaxations, each timed with HIP events on the solver's
+    # own stream around repeated solves (the kernels of the timed steps)
+    out = {"sec_step": elapsed / steps, "launches": s.launches_per_step()}
+    if time_solve:
+        reps = max(3, min(20, int(0.5 / max(elapsed / steps / 5, 1e-4))))
+        out["ms_pressure"] = s.time_pressure_sweeps(reps)
+        out["ms_diffuse"] = s.time_diffuse_sweeps(diff, visc, dt, max(2, reps // 2))
+        if order == 0:
+            out["policy"] = {"diffuse": s.gs_policy(0), "pressure": s.gs_policy(1)}
     s.close()
-    return elapsed / steps, ms_solve, launches
+    return out
+
+
+def solve_roofline(order: int, n: int, K: int, ms: float, kind: str, fields: int, policy) -> dict:
+    """Roofline of one relaxation solve (the unit of work: K sweeps over
+    `fields` fields of n^2 cells, 12 algorithmic bytes per cell per sweep)."""
+    cells = n * n
+    alg = 12 * cells * K * fields
+    gbs = alg / (ms * 1e-3) / 1e9
+    traffic = profile_value(order, n, K, f"traffic_bytes_per_{kind}_solve")
+    if order == 0:
+        name = f"k_gs_unit<{policy['tile_cols']},{policy['sweeps_per_unit']},{policy['lag']},{kind}>"
+    else:
+        name = f"k_jacobi_wave ({kind})"
+    r = {"kernel": name, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs <= HBM_PEAK_GBS else None,
+         "traffic": int(traffic) if traffic else None,
+         "hbm_frac_counters": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+         "unit_of_work": f"one {kind} solve: {K} sweeps x {fields} field(s) x {cells} cells x 12 B = "
+                         f"{alg / 1e9:.2f} GB algorithmic, {ms:.3f} ms (HIP events); traffic = HBM bytes "
+                         "per solve by rocprofv3 counters"}
+    if gbs > HBM_PEAK_GBS:
+        r["algorithmic_over_peak"] = round(gbs / HBM_PEAK_GBS, 3)
+        r["note"] = "temporal blocking: the algorithmic bytes are not moved; frac is given only by counters"
+    if order == 0:
+        lv = policy["levels"]
+        r["latency_model"] = {"levels": lv, "us_per_level": round(ms * 1e3 / lv, 2),
+                              "model": "the solve is a chain of dependent level launches: time = levels x "
+                                       "(slowest unit of the level + launch gap)"}
+    return r
 
 
 def run_batch(count: int, steps: int, warmup: int):
@@ -185,9 +222,16 @@ def cpu_baseline(n: int, kd: int, kp: int, params, sample_n: int):
     diff, visc, dt = params
     rho, u, v = W.vortex_state(Rs, seed=3)
     z = np.zeros((Rs, Rs), np.float32)
-    t0 = time.perf_counter()
-    O.solve(rho, z, diff, u, v, z, z, visc, dt, kd, kp, oracle.GS_LEX)
-    sec = time.perf_counter() - t0
+    # pinned to one host core for the sample (the reference solver is single-threaded)
+    cores = os.sched_getaffinity(0)
+    pin = min(cores)
+    os.sched_setaffinity(0, {pin})
+    try:
+        t0 = time.perf_counter()
+        O.solve(rho, z, diff, u, v, z, z, visc, dt, kd, kp, oracle.GS_LEX)
+        sec = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, cores)
     scale = (n * n) / float(sample_n * sample_n)
     cpu_model = "unknown"
     try:
@@ -198,10 +242,13 @@ def cpu_baseline(n: int, kd: int, kp: int, params, sample_n: int):
     except OSError:
         pass
     return {"value": round(1.0 / (sec * scale), 6), "unit": "steps/s", "cores": 1, "kind": "port",
-            "cpu_model": cpu_model, "host_cpus": os.cpu_count(),
+            "cpu_model": cpu_model, "host_cpus": os.cpu_count(), "pinned_core": pin,
+            "scaled": scale != 1.0,
             "sample": f"one full step at {sample_n}x{sample_n}+ring (K_d=K_p={kd}, same dt/viscosity/diffusion, "
-                      f"seeded 64-vortex start) took {sec:.2f} s on one host thread; scaled x{scale:g} by cell "
-                      f"count to {n}x{n}; oracle/fs2d_oracle.c, bit-identical to the reference CPU solver"}
+                      f"seeded 64-vortex start) took {sec:.2f} s on host core {pin} (affinity pinned); "
+                      + (f"SCALED x{scale:g} by cell count to {n}x{n} (every stage is linear in the cells), "
+                         if scale != 1.0 else "")
+                      + "oracle/fs2d_oracle.c, bit-identical to the reference CPU solver"}
 
 
 def run_render(R: int, state, reps: int = 20) -> dict:
@@ -260,12 +307,21 @@ def main() -> int:
     state = W.vortex_state(R, seed=3)
     order = 0 if args.order == "gs" else 1
 
-    sec_step, ms_solve, launches = run_gpu(order, n, K, K, args.steps, args.warmup, state, params)
+    g = run_gpu(order, n, K, K, args.steps, args.warmup, state, params)
+    sec_step, launches = g["sec_step"], g["launches"]
     cells = n * n
     bpc = bytes_per_cell_step(K, K, params[0] != 0, params[1] != 0)
     step_gbs = bpc * cells / sec_step / 1e9
-    solve_bytes = 12 * cells * K  # read p, read d, write p per cell per sweep
-    solve_gbs = solve_bytes / (ms_solve * 1e-3) / 1e9
+    pol = g.get("policy") or {"diffuse": None, "pressure": None}
+    d_fields = (1 if params[0] != 0 else 0) + (2 if params[1] != 0 else 0)
+    rl_d = solve_roofline(order, n, K, g["ms_diffuse"], "diffuse", d_fields, pol["diffuse"])
+    rl_p = solve_roofline(order, n, K, g["ms_pressure"], "pressure", 1, pol["pressure"])
+    # the dominant kernel of a step: one diffuse solve (three fields) against two pressure solves
+    per_step = {"diffuse": g["ms_diffuse"], "pressure": 2 * g["ms_pressure"]}
+    dom, other = (rl_d, rl_p) if per_step["diffuse"] >= per_step["pressure"] else (rl_p, rl_d)
+    dom = dict(dom)
+    dom["share_of_step"] = round(max(per_step.values()) / (sec_step * 1e3), 4)
+    dom["other_solve"] = dict(other, share_of_step=round(min(per_step.values()) / (sec_step * 1e3), 4))
     out = {
         "metric": f"steps/s (plain stable-fluids step, {n}^2 interior, K={K})",
         "value": round(1.0 / sec_step, 3),
@@ -288,27 +344,26 @@ def main() -> int:
         "bytes_per_cell_step": bpc,
         "launches_per_step": launches,
         "stream_copy_gbs": round(fs2d_amd_copy_bandwidth(), 1),
-        "roofline": {
-            "kernel": "k_gs_unit<64,8,4,pressure>" if order == 0 else "k_jacobi_block<4>",
-            "bound": "hbm",
-            "achieved": round(solve_gbs, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(solve_gbs / HBM_PEAK_GBS, 4),
-            "traffic": committed_traffic(order, n, K),
-            "unit_of_work": f"one pressure solve: {K} sweeps x {cells} cells x 12 B, {ms_solve:.3f} ms; "
-                            "achieved and traffic are per solve",
-        },
+        "roofline": dom,
     }
+    if order == 0:
+        out["gs_launch_policy"] = dict(pol, chosen="timed on the first solve of each kind (tile columns, sweeps "
+                                                   "per unit, lag); identical results for every choice")
     if order == 0 and not args.no_jacobi:
-        js, jms, jl = run_gpu(1, n, K, K, args.steps, args.warmup, state, params)
+        j = run_gpu(1, n, K, K, args.steps, args.warmup, state, params)
+        js = j["sec_step"]
         jg = bpc * cells / js / 1e9
+        jp = solve_roofline(1, n, K, j["ms_pressure"], "pressure", 1, None)
+        valu = profile_value(1, n, K, "valu_instructions_per_pressure_solve")
+        if valu:
+            jp["valu_issue_frac"] = round(valu / (j["ms_pressure"] * 1e-3) / VALU_ISSUE_PEAK, 4)
         out["jacobi"] = {"value": round(1.0 / js, 3), "unit": "steps/s", "ms_per_step": round(js * 1e3, 3),
                          "cell_updates_per_s": round(cells / js, 1),
-                         "step_hbm_gbs_algorithmic": round(jg, 1), "step_roofline_frac": round(jg / HBM_PEAK_GBS, 4),
-                         "pressure_solve_ms": round(jms, 4),
-                         "pressure_solve_gbs_algorithmic": round(12 * cells * K / (jms * 1e-3) / 1e9, 1),
-                         "launches_per_step": jl, "kernel": "k_jacobi_wave (depth autotuned in {1,2,4,8})"}
+                         "step_gbs_algorithmic": round(jg, 1),
+                         "step_algorithmic_over_peak": round(jg / HBM_PEAK_GBS, 3),
+                         "pressure_solve": jp, "launches_per_step": j["launches"],
+                         "note": "temporally blocked sweeps: algorithmic bytes over the peak is not a roofline "
+                                 "fraction; pressure_solve.hbm_frac_counters is"}
     if not args.no_render:
         out["render"] = run_render(R, state)
     del state
@@ -319,10 +374,14 @@ def main() -> int:
         for name, o in (("gs", 0), ("jacobi", 1)):
             if o == 1 and args.no_jacobi:
                 continue
-            sec8, _, _ = run_gpu(o, n8, K, K, 5, max(args.warmup, 13), st8, params, time_solve=False)
+            sec8 = run_gpu(o, n8, K, K, 5, max(args.warmup, 13), st8, params, time_solve=False)["sec_step"]
             g8 = bpc * n8 * n8 / sec8 / 1e9
             big[name] = {"value": round(1.0 / sec8, 3), "unit": "steps/s", "ms_per_step": round(sec8 * 1e3, 3),
-                         "step_hbm_gbs_algorithmic": round(g8, 1), "step_roofline_frac": round(g8 / HBM_PEAK_GBS, 4)}
+                         "step_gbs_algorithmic": round(g8, 1)}
+            if g8 <= HBM_PEAK_GBS:
+                big[name]["step_roofline_frac"] = round(g8 / HBM_PEAK_GBS, 4)
+            else:
+                big[name]["step_algorithmic_over_peak"] = round(g8 / HBM_PEAK_GBS, 3)
         del st8
         out["n8192"] = big
     if not args.no_batch:

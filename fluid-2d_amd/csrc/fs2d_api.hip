@@ -15,7 +15,9 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -26,6 +28,10 @@ namespace {
 thread_local std::string g_create_error;
 
 enum { F_RHO, F_U, F_V, F_RS, F_US, F_VS, F_X0, F_T1, F_T2, F_D, F_P, F_SCRATCH, F_X0U, F_X0V, F_UN, F_VN, F_COUNT };
+
+struct GsPolicy {
+    int tw, M, lag;  // tile columns, sweeps per unit (0: one-sweep tile kernel), lag
+};
 
 }  // namespace
 
@@ -39,10 +45,12 @@ struct fs2d_handle {
     float* f[F_COUNT] = {};
     unsigned* max_bits = nullptr;
     unsigned* speed_slots = nullptr;  // kSpeedSlots * kSlotStride words, zero between uses
-    std::map<std::pair<int, int>, GsPlan> gs_plans;  // by (sweep count, tile columns)
-    int gs_tw = 0;  // tile columns; 0: 128 for diffuse solves, 64 for pressure solves
-    int gs_block = 8;
-    int gs_lag = 4;
+    std::map<std::tuple<int, int, int, int>, GsPlan> gs_plans;  // by (sweep count, tile columns, block, lag)
+    // Gauss-Seidel launch policy: fixed by the parameters, or (0) chosen per
+    // (kind, sweep count) by gs_policy_for; gs_block 1 selects the one-sweep
+    // tile kernel, anything else the unit kernels.
+    int gs_tw = 0, gs_block = 0, gs_lag = 0;
+    std::map<std::pair<int, int>, GsPolicy> gs_policy;  // (kind, K) -> policy in use
     int jacobi_kernel = FS2D_JACOBI_WAVE;
     std::map<std::pair<int, int>, int> jacobi_depth;  // (kind, K) -> sweeps per launch, when tuned
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
@@ -98,29 +106,146 @@ Scalars scalars(int R, float dt) {
     return s;
 }
 
-// Tile columns of a Gauss-Seidel solve: the handle's setting, or by kind --
-// the diffuses' dense levels run faster with fewer, wider units, the pressure
-// solve's latency-bound levels with narrower ones (identical results).
-int gs_tile_cols_for(const fs2d_handle* h, int kind) {
-    if (h->gs_tw) return h->gs_tw;
-    return (kind == SWEEP_DIFFUSE && h->gs_block != 2) ? 128 : 64;
-}
+bool gs_units(const fs2d_handle* h) { return h->gs_block != 1; }
 
-hipError_t gs_plan_for(fs2d_handle* h, int K, int kind, const GsPlan** out) {
-    const int tw = gs_tile_cols_for(h, kind);
-    auto it = h->gs_plans.find({K, tw});
+hipError_t gs_plan_of(fs2d_handle* h, int K, const GsPolicy& c, const GsPlan** out) {
+    const auto key = std::make_tuple(K, c.tw, c.M, c.lag);
+    auto it = h->gs_plans.find(key);
     if (it == h->gs_plans.end()) {
         GsPlan plan{};
-        hipError_t e = (h->gs_block > 1) ? gs_unit_plan_create(&plan, h->R, K, tw, h->gs_block, h->gs_lag)
-                                         : gs_plan_create(&plan, h->R, K, tw);
+        hipError_t e = c.M > 0 ? gs_unit_plan_create(&plan, h->R, K, c.tw, c.M, c.lag) : gs_plan_create(&plan, h->R, K, c.tw);
         if (e != hipSuccess) {
             gs_plan_destroy(&plan);
             return e;
         }
-        it = h->gs_plans.emplace(std::make_pair(K, tw), plan).first;
+        it = h->gs_plans.emplace(key, plan).first;
     }
     *out = &it->second;
     return hipSuccess;
+}
+
+// Gauss-Seidel launch policy of a solve kind and sweep count (the reference
+// fixes its launch shape, kernel_launcher.hpp:10-30; every policy here gives
+// identical results).  Parameters left at 0 are chosen on the first solve of
+// each (kind, K): every candidate -- tile columns {64, 128} x sweeps per unit
+// {4, 8} x lag {4, 8}, within the fixed parameters -- relaxes zero-started
+// scratch iterates against the resident fields (three fields for a diffuse,
+// as the step relaxes them together), timed with HIP events (best of two
+// after a warm-up), and the fastest is kept, also process-wide for the same
+// (device, rows, K, kind, fixed parameters).  A stream under capture cannot
+// be timed: the default (128 columns for diffuses, 64 for pressure; 8 sweeps;
+// lag 4) serves it and is not kept.  Plans are kept until the handle is
+// destroyed (a captured step may hold any of them).
+std::mutex g_policy_mu;
+std::map<std::tuple<int, int, int, int, int, int, int>, GsPolicy> g_policy_cache;
+
+hipError_t gs_policy_for(fs2d_handle* h, int K, int kind, GsPolicy* out) {
+    const auto hkey = std::make_pair(kind, K);
+    auto it = h->gs_policy.find(hkey);
+    if (it != h->gs_policy.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    GsPolicy def{};
+    def.M = h->gs_block == 1 ? 0 : (h->gs_block ? h->gs_block : 8);
+    def.tw = h->gs_tw ? h->gs_tw : ((kind == SWEEP_DIFFUSE && def.M != 2) ? 128 : 64);
+    def.lag = def.M == 0 ? 1 : (h->gs_lag ? h->gs_lag : 4);
+    std::vector<GsPolicy> cand;
+    for (int tw : {64, 128})
+        for (int M : {4, 8})
+            for (int lag : {4, 8}) {
+                GsPolicy c{h->gs_tw ? h->gs_tw : tw, h->gs_block ? (h->gs_block == 1 ? 0 : h->gs_block) : M,
+                           h->gs_lag ? h->gs_lag : lag};
+                if (c.M == 0) c.lag = 1;
+                if (c.M == 2 && c.tw == 128) continue;
+                bool dup = false;
+                for (const auto& d : cand) dup = dup || (d.tw == c.tw && d.M == c.M && d.lag == c.lag);
+                if (!dup) cand.push_back(c);
+            }
+    if (cand.size() <= 1 || K <= 0) {
+        *out = cand.size() == 1 ? cand[0] : def;
+        h->gs_policy.emplace(hkey, *out);
+        return hipSuccess;
+    }
+    const auto gkey = std::make_tuple(h->device, h->R, K, kind, h->gs_tw, h->gs_block, h->gs_lag);
+    {
+        std::lock_guard<std::mutex> lock(g_policy_mu);
+        auto g = g_policy_cache.find(gkey);
+        if (g != g_policy_cache.end()) {
+            *out = g->second;
+            h->gs_policy.emplace(hkey, *out);
+            return hipSuccess;
+        }
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(h->stream, &cs);
+    if (e != hipSuccess) return e;
+    if (cs != hipStreamCaptureStatusNone) {
+        *out = def;
+        return hipSuccess;
+    }
+    const int nf = kind == SWEEP_DIFFUSE ? 3 : 1;
+    float* tmp[3] = {nullptr, nullptr, nullptr};
+    auto release = [&]() {
+        for (float* q : tmp)
+            if (q) (void)hipFree(q);
+    };
+    for (int f = 0; f < nf; ++f)
+        if ((e = hipMalloc(&tmp[f], h->field_bytes)) != hipSuccess) return release(), e;
+    SweepSet set{};
+    set.count = nf;
+    const int modes[3] = {BND_CONT, BND_OPP_H, BND_OPP_V};
+    const int rhs[3] = {F_RHO, F_U, F_V};
+    for (int f = 0; f < nf; ++f)
+        set.a[f] = kind == SWEEP_DIFFUSE
+                       ? sweep_args(tmp[f], h->f[rhs[f]], h->R, h->P, SWEEP_DIFFUSE, 1.0f, 5.0f, modes[f], K, 0)
+                       : sweep_args(tmp[f], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 0);
+    float best_ms = 0.f;
+    GsPolicy best = def;
+    for (size_t c = 0; c < cand.size(); ++c) {
+        const GsPlan* plan = nullptr;
+        if ((e = gs_plan_of(h, K, cand[c], &plan)) != hipSuccess) return release(), e;
+        SweepSet run = set;
+        if (plan->M == 0 && nf > 1) run.count = 1;  // the tile kernel relaxes one field per launch sequence
+        float t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            for (int f = 0; f < nf && e == hipSuccess; ++f) e = launch_fill(h->stream, tmp[f], 0u, h->field_bytes / 4);
+            if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+            for (int f = 0; f < nf && e == hipSuccess; ++f) {
+                if (plan->M == 0) {
+                    e = launch_gs_solve(h->stream, *plan, set.a[f]);
+                } else {
+                    e = launch_gs_solve_set(h->stream, *plan, run);
+                    break;
+                }
+            }
+            if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+            if (e != hipSuccess) return release(), e;
+            if (rep == 1 || (rep > 1 && ms < t)) t = ms;
+        }
+        if (c == 0 || t < best_ms) {
+            best_ms = t;
+            best = cand[c];
+        }
+    }
+    release();
+    {
+        std::lock_guard<std::mutex> lock(g_policy_mu);
+        g_policy_cache.emplace(gkey, best);
+    }
+    h->gs_policy.emplace(hkey, best);
+    *out = best;
+    return hipSuccess;
+}
+
+hipError_t gs_plan_for(fs2d_handle* h, int K, int kind, const GsPlan** out) {
+    GsPolicy c{};
+    hipError_t e = gs_policy_for(h, K, kind, &c);
+    if (e != hipSuccess) return e;
+    return gs_plan_of(h, K, c, out);
 }
 
 constexpr int kJacobiDefaultDepth = 4;
@@ -217,7 +342,7 @@ int diffuse(fs2d_handle* h, float*& x, int mode, float rate, float dt, int K) {
 // kernel and Jacobi take a zero-filled p; the one-sweep tile kernel reads
 // zeros itself.
 bool pressure_needs_fill(fs2d_handle* h) {
-    return !(h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block <= 1);
+    return !(h->prm.sweep_order == FS2D_ORDER_GS_LEX && !gs_units(h));
 }
 
 int project(fs2d_handle* h, int K) {
@@ -302,7 +427,7 @@ int step(fs2d_handle* h, float diff, float visc, float dt) {
     const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
     const int R = h->R, P = h->P;
     int rc;
-    if (h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block > 1) return step_shared_diffuse(h, diff, visc, dt);
+    if (h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h)) return step_shared_diffuse(h, diff, visc, dt);
     // density: add source, diffuse, scatter-advect (:16-18)
     FS2D_HIP(h, launch_add_sources(h->stream, h->rho(), h->f[F_X0], h->f[F_RS], R, P, dt));
     if ((rc = diffuse(h, h->rho(), BND_CONT, diff, dt, Kd))) return rc;
@@ -477,8 +602,8 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     h->field_bytes = sizeof(float) * static_cast<size_t>(h->R) * h->P;
     h->prm = prm;
     h->gs_tw = prm.gs_tile_cols;
-    h->gs_block = prm.gs_block ? prm.gs_block : 8;
-    h->gs_lag = prm.gs_lag ? prm.gs_lag : 4;
+    h->gs_block = prm.gs_block;
+    h->gs_lag = prm.gs_lag;
     h->jacobi_kernel = prm.jacobi_kernel ? prm.jacobi_kernel : FS2D_JACOBI_WAVE;
     auto bail = [&](hipError_t e, const char* what) {
         std::string msg = std::string("fs2d_hip_create: ") + what + ": " + hipGetErrorString(e);
@@ -694,6 +819,66 @@ int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
     return FS2D_OK;
 }
 
+int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms) {
+    if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_diffuse_sweeps: bad argument");
+    const int Kd = h->prm.diffuse_iterations;
+    const int R = h->R, P = h->P;
+    const size_t n4 = h->field_bytes / 16;
+    // right-hand sides and iterates are copies of the state, which stays untouched
+    const int pairs[6][2] = {{F_X0, F_RHO}, {F_X0U, F_U}, {F_X0V, F_V}, {F_T1, F_RHO}, {F_UN, F_U}, {F_VN, F_V}};
+    for (const auto& pr : pairs)
+        FS2D_HIP(h, launch_copy4(h->stream, reinterpret_cast<float4*>(h->f[pr[0]]),
+                                 reinterpret_cast<const float4*>(h->f[pr[1]]), n4));
+    const float ad = dt * cells(R) * diff, av = dt * cells(R) * visc;
+    const float dd = 1.f + 4.f * ad, dv = 1.f + 4.f * av;
+    auto once = [&]() -> int {
+        if (Kd <= 0) return FS2D_OK;
+        if (h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h)) {  // as the step relaxes them
+            SweepSet set{};
+            set.count = 3;
+            set.a[0] = sweep_args(h->f[F_T1], h->f[F_X0], R, P, SWEEP_DIFFUSE, ad, dd, BND_CONT, Kd, 0);
+            set.a[1] = sweep_args(h->f[F_UN], h->f[F_X0U], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_H, Kd, 0);
+            set.a[2] = sweep_args(h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_V, Kd, 0);
+            const GsPlan* plan = nullptr;
+            FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
+            FS2D_HIP(h, launch_gs_solve_set(h->stream, *plan, set));
+            return FS2D_OK;
+        }
+        int rc;
+        if ((rc = linear_solve(h, h->f[F_T1], h->f[F_X0], SWEEP_DIFFUSE, ad, dd, BND_CONT, Kd, 0))) return rc;
+        if ((rc = linear_solve(h, h->f[F_UN], h->f[F_X0U], SWEEP_DIFFUSE, av, dv, BND_OPP_H, Kd, 0))) return rc;
+        return linear_solve(h, h->f[F_VN], h->f[F_X0V], SWEEP_DIFFUSE, av, dv, BND_OPP_V, Kd, 0);
+    };
+    int rc = once();  // warm
+    if (rc) return rc;
+    FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < reps; ++i)
+        if ((rc = once())) return rc;
+    FS2D_HIP(h, hipEventRecord(h->ev1, h->stream));
+    FS2D_HIP(h, hipEventSynchronize(h->ev1));
+    float total = 0.f;
+    FS2D_HIP(h, hipEventElapsedTime(&total, h->ev0, h->ev1));
+    *ms = total / reps;
+    return FS2D_OK;
+}
+
+int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels) {
+    if (!h || (kind != 0 && kind != 1)) return fail(h, FS2D_ERR_ARGUMENT, "gs_policy: bad argument");
+    if (h->prm.sweep_order != FS2D_ORDER_GS_LEX) return fail(h, FS2D_ERR_STATE, "gs_policy: Gauss-Seidel order only");
+    const int K = kind == 0 ? h->prm.diffuse_iterations : h->prm.pressure_iterations;
+    if (K <= 0) return fail(h, FS2D_ERR_STATE, "gs_policy: no sweeps of that kind");
+    const int sk = kind == 0 ? SWEEP_DIFFUSE : SWEEP_PRESSURE;
+    GsPolicy c{};
+    FS2D_HIP(h, gs_policy_for(h, K, sk, &c));
+    const GsPlan* plan = nullptr;
+    FS2D_HIP(h, gs_plan_of(h, K, c, &plan));
+    if (tile_cols) *tile_cols = c.tw;
+    if (sweeps_per_unit) *sweeps_per_unit = c.M == 0 ? 1 : c.M;
+    if (lag) *lag = c.lag;
+    if (levels) *levels = plan->levels;
+    return FS2D_OK;
+}
+
 int fs2d_hip_copy_bandwidth(long long bytes, int reps, float* gbs) {
     if (bytes < 1024 || reps < 1 || !gbs) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_copy_bandwidth: bad argument");
     const size_t n4 = static_cast<size_t>(bytes) / 16;
@@ -736,8 +921,8 @@ int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
     const int d = solve_launches(Kd, SWEEP_DIFFUSE), p = solve_launches(Kp, SWEEP_PRESSURE);
     if (d < 0 || p < 0) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
     // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather
-    const bool shared = h->prm.sweep_order == FS2D_ORDER_GS_LEX && h->gs_block > 1;
-    *launches = 3 + (shared ? d : 3 * d) + 1 + 2 * (2 + p) + 1;
+    const bool shared = h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h);
+    *launches = 3 + (shared ? d : 3 * d) + scatter_launches(h->R) + 2 * (2 + p) + 1;
     return FS2D_OK;
 }
 

@@ -93,6 +93,7 @@ hipError_t launch_max_speed(hipStream_t s, unsigned* out_bits, const float* u, c
 hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, const float* u, const float* v, int R,
                                   int P, float dt0, const unsigned* max_bits, void* sort_scratch);
 size_t scatter_sort_bytes(int R);
+int scatter_launches(int R);  // kernels launch_scatter_density enqueues with sort_scratch
 hipError_t launch_gather_velocity(hipStream_t s, float* u1, float* v1, const float* u0, const float* v0,
                                   int R, int P, float dt0);
 // p_zero (may be null): also zero this pressure iterate, ring included.

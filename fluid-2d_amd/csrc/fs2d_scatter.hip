@@ -496,6 +496,19 @@ __global__ __launch_bounds__(NT) void k_scatter_merge(float* __restrict__ out, c
 
 }  // namespace
 
+// Bits of the sort keys: targets 0 .. (R - 1)^2 (the last one for skipped sources).
+static int sort_key_bits(int R) {
+    const unsigned long long nb = (unsigned long long)(R - 1) * (unsigned long long)(R - 1);
+    int bits = 1;
+    while ((1ull << bits) <= nb) ++bits;
+    return bits;
+}
+
+int scatter_launches(int R) {
+    const int passes = (sort_key_bits(R) + DB - 1) / DB;
+    return 3 + 3 * passes;  // window, keys, (passes - 1) histograms, passes x (scan, scatter), bounds, merge
+}
+
 size_t scatter_sort_bytes(int R) {
     const size_t n = static_cast<size_t>(R - 2);
     const size_t N = n * n;
@@ -526,9 +539,7 @@ hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, 
     const unsigned cell_blocks = static_cast<unsigned>(std::min<long long>((N + NT - 1) / NT, kSortBlocks));
     hipLaunchKernelGGL(k_sort_keys, dim3(chunk_blocks), dim3(NT), 0, s, items0, hist, bounds, u, v, R, P, dt0, chunks,
                        max_bits);
-    // key bits: targets 0 .. (R - 1)^2 (the last one for skipped sources)
-    int bits = 1;
-    while ((1ull << bits) <= (unsigned long long)nb) ++bits;
+    const int bits = sort_key_bits(R);
     unsigned long long* src = items0;
     unsigned long long* dst = items1;
     for (int shift = 0; shift < bits; shift += DB) {

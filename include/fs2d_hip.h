@@ -70,11 +70,13 @@ typedef struct fs2d_params {
     int pressure_iterations; /* K_p; the reference's literal is 20 (src/fluid_solver_cpu.cpp:25,30)    */
     int sweep_order;         /* enum fs2d_sweep_order                                                   */
     int jacobi_block;        /* Jacobi sweeps fused per launch (temporal block depth); 0 = autotuned    */
-    int gs_tile_cols;        /* Gauss-Seidel tile width in cells: 64 or 128 (128 takes gs_block 1, 4, 8);
-                              * 0 = 128 for the diffuse solves and 64 for the pressure solves
-                              * (64 for both with gs_block 2); identical results            */
-    int gs_block;            /* Gauss-Seidel sweeps per tile (temporal block): 1, 2, 4 or 8; 0 = default */
-    int gs_lag;              /* steps each sweep trails the previous one in a tile: 1, 4 or 8; 0 = default */
+    int gs_tile_cols;        /* Gauss-Seidel tile width in cells: 64 or 128 (128 takes gs_block 1, 4, 8) */
+    int gs_block;            /* Gauss-Seidel sweeps per tile (temporal block): 1, 2, 4 or 8              */
+    int gs_lag;              /* steps each sweep trails the previous one in a tile: 1, 4 or 8
+                              * Each of these three left at 0 is chosen per solve kind and sweep count
+                              * on the first solve, by timing the candidates (64/128 columns, 4/8
+                              * sweeps, lag 4/8) on scratch fields; cached per process.  All choices
+                              * give identical results.                                             */
     int jacobi_kernel;       /* enum fs2d_jacobi_kernel; 0 = default (register strips)                   */
     int launch_mode;         /* enum fs2d_launch_mode; 0 = graph replay                                  */
 } fs2d_params;
@@ -148,6 +150,17 @@ void* fs2d_hip_stream(fs2d_handle* h);
  * milliseconds, measured with HIP events on the handle's stream.  The state's
  * velocity is not modified. */
 int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms_per_solve);
+/* Enqueue the three diffuse solves of one step (density, u, v: K_d sweeps
+ * each, shared level launches in the Gauss-Seidel order as the step runs
+ * them) `reps` times on copies of the state, and return the mean time of one
+ * step's diffuses in milliseconds, measured with HIP events on the handle's
+ * stream.  The state (density, velocity, pressure) is not modified. */
+int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms_per_solve);
+/* The Gauss-Seidel launch policy of the diffuse (kind 0, K_d sweeps) or
+ * pressure (kind 1, K_p sweeps) solves: tile columns, sweeps per unit (1: the
+ * one-sweep tile kernel), lag, and level launches per solve.  Chooses it first
+ * if no solve of that kind has run (see gs_tile_cols, gs_block, gs_lag). */
+int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels);
 /* On-device copy bandwidth: a float4 copy kernel over `bytes` (read + write
  * counted), timed with HIP events over `reps` copies; the achievable HBM
  * figure beside the 8 TB/s peak. */

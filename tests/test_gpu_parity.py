@@ -324,18 +324,33 @@ def test_wide_tiles_vs_oracle(R, gsb, lag, oracle_lib):
         assert_same(got[f], want[f], f"wide step {name} R={R} gs_block={gsb} lag={lag}")
 
 
-@pytest.mark.parametrize("gsb,tw", [(8, 0), (8, 64), (8, 128), (4, 0), (2, 0)])
-def test_tile_width_by_solve_kind(gsb, tw):
-    """gs_tile_cols = 0 gives the diffuses 128-column units and the pressure
-    solves 64-column ones (64 for both with gs_block 2): the launches of a
-    step equal the level counts of the schedule model's unit plans."""
+def scatter_launches(R):
+    """Kernels of a step's density scatter: the window path, and the sorted
+    path's keys, histograms, scans, scatters (one per 9-bit digit of the
+    (R - 1)^2 + 1 target keys), bounds and merge."""
+    passes = -(-((R - 1) ** 2).bit_length() // 9)
+    return 3 + 3 * passes
+
+
+@pytest.mark.parametrize("gsb,tw,lag", [(8, 0, 4), (8, 64, 0), (8, 128, 8), (4, 0, 0), (2, 0, 0), (0, 0, 0)])
+def test_launch_policy_and_launch_count(gsb, tw, lag):
+    """The Gauss-Seidel launch policy of each solve kind keeps the fixed
+    parameters and chooses the others among the candidates (64 or 128
+    columns, 4 or 8 sweeps per unit, lag 4 or 8); a step's launches equal the
+    level counts of the schedule model's unit plans for the chosen policy."""
     from schedule_models import gs_unit_plan
     R, K = 258, 20
-    tw_d = tw or (64 if gsb == 2 else 128)
-    tw_p = tw or 64
-    d, p = len(gs_unit_plan(R, K, tw_d, gsb)), len(gs_unit_plan(R, K, tw_p, gsb))
-    with fs2d_amd.Solver(R, diffuse_iterations=K, pressure_iterations=K, gs_block=gsb, gs_tile_cols=tw) as s:
-        assert s.launches_per_step() == 3 + d + 1 + 2 * (2 + p) + 1, (d, p)
+    with fs2d_amd.Solver(R, diffuse_iterations=K, pressure_iterations=K, gs_block=gsb, gs_tile_cols=tw,
+                         gs_lag=lag) as s:
+        pol = [s.gs_policy(kind) for kind in (0, 1)]
+        launches = s.launches_per_step()
+    for q in pol:
+        assert q["tile_cols"] == tw if tw else q["tile_cols"] in ((64,) if gsb == 2 else (64, 128)), q
+        assert q["sweeps_per_unit"] == gsb if gsb else q["sweeps_per_unit"] in (4, 8), q
+        assert q["lag"] == lag if lag else q["lag"] in (4, 8), q
+    d, p = (len(gs_unit_plan(R, K, q["tile_cols"], q["sweeps_per_unit"])) for q in pol)
+    assert (pol[0]["levels"], pol[1]["levels"]) == (d, p)
+    assert launches == 3 + d + scatter_launches(R) + 2 * (2 + p) + 1, (d, p, pol)
 
 
 # ------------------------------------------------ the reference's own outputs
@@ -458,8 +473,31 @@ def test_jacobi_autotuned_depth(jk, oracle_lib):
     for f, name in enumerate(("density", "u", "v")):
         assert_same(got[f], want[f], f"autotuned Jacobi {name} jk={jk}")
     # 3 adds, 3 diffuses of ceil(20/m_d), scatter, 2 x (div, ceil(40/m_p), grad), gather
-    possible = {3 + 3 * -(-20 // md) + 1 + 2 * (2 + -(-40 // mp)) + 1 for md in (1, 2, 4, 8) for mp in (1, 2, 4, 8)}
+    sc = scatter_launches(R)
+    possible = {3 + 3 * -(-20 // md) + sc + 2 * (2 + -(-40 // mp)) + 1 for md in (1, 2, 4, 8) for mp in (1, 2, 4, 8)}
     assert launches in possible
+
+
+@pytest.mark.parametrize("order", [GS, JAC])
+def test_solve_timers_leave_the_state(order, oracle_lib):
+    """The event-timed diffuse and pressure solves run on scratch fields: the
+    next step from the same state is still the oracle's step."""
+    R = 130
+    st = W.vortex_state(R, seed=9, n_vortices=16)
+    z = np.zeros((R, R), np.float32)
+    want = [x.copy() for x in st]
+    with fs2d_amd.Solver(R, diffuse_iterations=20, pressure_iterations=20, order=order) as s:
+        s.upload(*st)
+        s.set_sources(None, None, None)
+        ms_d = s.time_diffuse_sweeps(1e-4, 1e-5, 0.05, 3)
+        assert [np.array_equal(a, b) for a, b in zip(s.download(), st)] == [True] * 3
+        s.step_n(1, 1e-4, 1e-5, 0.05)
+        got = s.download()
+    oracle_lib.solve(want[0], z, 1e-4, want[1], want[2], z, z, 1e-5, 0.05, 20, 20,
+                     oracle.GS_LEX if order == GS else oracle.JACOBI)
+    assert ms_d > 0
+    for f, name in enumerate(("density", "u", "v")):
+        assert_same(got[f], want[f], f"step after the diffuse timer, {name}")
 
 
 def test_config2_one_step(oracle_lib):
