@@ -16,6 +16,8 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <exception>
+#include <new>
 #include <string>
 #include <tuple>
 #include <utility>
@@ -550,6 +552,21 @@ int copy_out(fs2d_handle* h, float* dst, const float* src, hipMemcpyKind kind) {
 
 bool field_ok(int field) { return field >= FS2D_FIELD_DENSITY && field <= FS2D_FIELD_VELOCITY_V; }
 
+// Exceptions thrown on the host (allocation in plan building, maps and
+// vectors) never cross the C ABI: they become an error code and message.
+template <class F>
+int guarded(fs2d_handle* h, const char* what, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(h, FS2D_ERR_HIP, std::string(what) + ": out of host memory");
+    } catch (const std::exception& e) {
+        return fail(h, FS2D_ERR_STATE, std::string(what) + ": " + e.what());
+    } catch (...) {
+        return fail(h, FS2D_ERR_STATE, std::string(what) + ": unexpected exception");
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -564,7 +581,7 @@ void fs2d_default_params(fs2d_params* p) {
     p->sweep_order = FS2D_ORDER_GS_LEX;
 }
 
-int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle** out) {
+static int fs2d_hip_create_impl(int rows, int cols, const fs2d_params* params, fs2d_handle** out) {
     if (!out) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: out is NULL");
     *out = nullptr;
     if (rows != cols)
@@ -595,6 +612,19 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: gs_tile_cols 128 takes gs_block 1, 4 or 8");
     if (prm.jacobi_kernel != 0 && prm.jacobi_kernel != FS2D_JACOBI_WAVE && prm.jacobi_kernel != FS2D_JACOBI_LDS)
         return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_create: jacobi_kernel must be 0, 1 (wave) or 2 (LDS)");
+    if (prm.sweep_order == FS2D_ORDER_GS_LEX) {
+        // The Gauss-Seidel plans enumerate tiles x sweep blocks on the host: bound
+        // them (the smallest candidate tiles, 64 x 64 with 4 sweeps per unit, or
+        // one sweep per tile) so plan building stays small and finite.
+        const long long n = rows - 2, K = std::max(prm.diffuse_iterations, prm.pressure_iterations);
+        const long long g = (n + K - 1 + 63) / 64;
+        const long long blocks = prm.gs_block == 1 ? K : (K + 3) / 4;
+        if (g * g * blocks > (1ll << 28))
+            return fail(nullptr, FS2D_ERR_ARGUMENT,
+                        "fs2d_hip_create: too many Gauss-Seidel sweeps for this grid (the unit plan would hold " +
+                            std::to_string(g * g * blocks) + " units; at most 2^28); use fewer iterations or the "
+                            "Jacobi order");
+    }
 
     fs2d_handle* h = new fs2d_handle();
     h->R = rows;
@@ -631,6 +661,10 @@ int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle**
     return FS2D_OK;
 }
 
+int fs2d_hip_create(int rows, int cols, const fs2d_params* params, fs2d_handle** out) {
+    return guarded(nullptr, "fs2d_hip_create", [&] { return fs2d_hip_create_impl(rows, cols, params, out); });
+}
+
 int fs2d_hip_destroy(fs2d_handle* h) {
     if (!h) return FS2D_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -657,7 +691,7 @@ const char* fs2d_hip_last_error(const fs2d_handle* h) { return h ? h->err.c_str(
 
 void* fs2d_hip_stream(fs2d_handle* h) { return h ? static_cast<void*>(h->stream) : nullptr; }
 
-int fs2d_hip_upload(fs2d_handle* h, const float* rho, const float* u, const float* v) {
+static int fs2d_hip_upload_impl(fs2d_handle* h, const float* rho, const float* u, const float* v) {
     if (!h || !rho || !u || !v) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_upload: NULL argument");
     int rc;
     h->speed_valid = false;
@@ -668,7 +702,11 @@ int fs2d_hip_upload(fs2d_handle* h, const float* rho, const float* u, const floa
     return FS2D_OK;
 }
 
-int fs2d_hip_upload_device(fs2d_handle* h, const float* rho, const float* u, const float* v) {
+int fs2d_hip_upload(fs2d_handle* h, const float* rho, const float* u, const float* v) {
+    return guarded(h, "fs2d_hip_upload", [&] { return fs2d_hip_upload_impl(h, rho, u, v); });
+}
+
+static int fs2d_hip_upload_device_impl(fs2d_handle* h, const float* rho, const float* u, const float* v) {
     if (!h || !rho || !u || !v) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_upload_device: NULL argument");
     int rc;
     h->speed_valid = false;
@@ -677,7 +715,11 @@ int fs2d_hip_upload_device(fs2d_handle* h, const float* rho, const float* u, con
     return copy_in(h, h->v(), v, hipMemcpyDeviceToDevice);
 }
 
-int fs2d_hip_set_sources(fs2d_handle* h, const float* rs, const float* us, const float* vs) {
+int fs2d_hip_upload_device(fs2d_handle* h, const float* rho, const float* u, const float* v) {
+    return guarded(h, "fs2d_hip_upload_device", [&] { return fs2d_hip_upload_device_impl(h, rho, u, v); });
+}
+
+static int fs2d_hip_set_sources_impl(fs2d_handle* h, const float* rs, const float* us, const float* vs) {
     if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_set_sources: NULL handle");
     const float* src[3] = {rs, us, vs};
     const int dst[3] = {F_RS, F_US, F_VS};
@@ -693,7 +735,11 @@ int fs2d_hip_set_sources(fs2d_handle* h, const float* rs, const float* us, const
     return FS2D_OK;
 }
 
-int fs2d_hip_step_n(fs2d_handle* h, int n, float diff, float visc, float dt) {
+int fs2d_hip_set_sources(fs2d_handle* h, const float* rs, const float* us, const float* vs) {
+    return guarded(h, "fs2d_hip_set_sources", [&] { return fs2d_hip_set_sources_impl(h, rs, us, vs); });
+}
+
+static int fs2d_hip_step_n_impl(fs2d_handle* h, int n, float diff, float visc, float dt) {
     if (!h || n < 0) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_step_n: bad argument");
     for (int i = 0; i < n; ++i) {
         int rc = advance(h, diff, visc, dt);
@@ -702,19 +748,31 @@ int fs2d_hip_step_n(fs2d_handle* h, int n, float diff, float visc, float dt) {
     return FS2D_OK;
 }
 
-int fs2d_hip_graph_count(fs2d_handle* h, int* count) {
+int fs2d_hip_step_n(fs2d_handle* h, int n, float diff, float visc, float dt) {
+    return guarded(h, "fs2d_hip_step_n", [&] { return fs2d_hip_step_n_impl(h, n, diff, visc, dt); });
+}
+
+static int fs2d_hip_graph_count_impl(fs2d_handle* h, int* count) {
     if (!h || !count) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_graph_count: bad argument");
     *count = static_cast<int>(h->graphs.size());
     return FS2D_OK;
 }
 
-int fs2d_hip_sync(fs2d_handle* h) {
+int fs2d_hip_graph_count(fs2d_handle* h, int* count) {
+    return guarded(h, "fs2d_hip_graph_count", [&] { return fs2d_hip_graph_count_impl(h, count); });
+}
+
+static int fs2d_hip_sync_impl(fs2d_handle* h) {
     if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_sync: NULL handle");
     FS2D_HIP(h, hipStreamSynchronize(h->stream));
     return FS2D_OK;
 }
 
-int fs2d_hip_download(fs2d_handle* h, float* rho, float* u, float* v) {
+int fs2d_hip_sync(fs2d_handle* h) {
+    return guarded(h, "fs2d_hip_sync", [&] { return fs2d_hip_sync_impl(h); });
+}
+
+static int fs2d_hip_download_impl(fs2d_handle* h, float* rho, float* u, float* v) {
     if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_download: NULL handle");
     int rc;
     if (rho && (rc = copy_out(h, rho, h->rho(), hipMemcpyDeviceToHost))) return rc;
@@ -724,7 +782,11 @@ int fs2d_hip_download(fs2d_handle* h, float* rho, float* u, float* v) {
     return FS2D_OK;
 }
 
-int fs2d_hip_download_device(fs2d_handle* h, float* rho, float* u, float* v) {
+int fs2d_hip_download(fs2d_handle* h, float* rho, float* u, float* v) {
+    return guarded(h, "fs2d_hip_download", [&] { return fs2d_hip_download_impl(h, rho, u, v); });
+}
+
+static int fs2d_hip_download_device_impl(fs2d_handle* h, float* rho, float* u, float* v) {
     if (!h) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_download_device: NULL handle");
     int rc;
     if (rho && (rc = copy_out(h, rho, h->rho(), hipMemcpyDeviceToDevice))) return rc;
@@ -733,7 +795,11 @@ int fs2d_hip_download_device(fs2d_handle* h, float* rho, float* u, float* v) {
     return FS2D_OK;
 }
 
-int fs2d_hip_download_pressure(fs2d_handle* h, float* p) {
+int fs2d_hip_download_device(fs2d_handle* h, float* rho, float* u, float* v) {
+    return guarded(h, "fs2d_hip_download_device", [&] { return fs2d_hip_download_device_impl(h, rho, u, v); });
+}
+
+static int fs2d_hip_download_pressure_impl(fs2d_handle* h, float* p) {
     if (!h || !p) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_download_pressure: NULL argument");
     if (h->pressure_zero) {
         std::memset(p, 0, sizeof(float) * static_cast<size_t>(h->R) * h->R);
@@ -745,7 +811,11 @@ int fs2d_hip_download_pressure(fs2d_handle* h, float* p) {
     return FS2D_OK;
 }
 
-int fs2d_hip_solve(fs2d_handle* h, float* rho, const float* rho_src, float diff, float* u, float* v,
+int fs2d_hip_download_pressure(fs2d_handle* h, float* p) {
+    return guarded(h, "fs2d_hip_download_pressure", [&] { return fs2d_hip_download_pressure_impl(h, p); });
+}
+
+static int fs2d_hip_solve_impl(fs2d_handle* h, float* rho, const float* rho_src, float diff, float* u, float* v,
                    const float* u_src, const float* v_src, float visc, float dt) {
     if (!h || !rho || !rho_src || !u || !v || !u_src || !v_src)
         return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_solve: NULL argument");
@@ -756,12 +826,17 @@ int fs2d_hip_solve(fs2d_handle* h, float* rho, const float* rho_src, float diff,
     return fs2d_hip_download(h, rho, u, v);
 }
 
+int fs2d_hip_solve(fs2d_handle* h, float* rho, const float* rho_src, float diff, float* u, float* v,
+                   const float* u_src, const float* v_src, float visc, float dt) {
+    return guarded(h, "fs2d_hip_solve", [&] { return fs2d_hip_solve_impl(h, rho, rho_src, diff, u, v, u_src, v_src, visc, dt); });
+}
+
 // ---- stages ------------------------------------------------------------------
 static float*& field_ref(fs2d_handle* h, int field) {
     return field == FS2D_FIELD_DENSITY ? h->rho() : (field == FS2D_FIELD_VELOCITY_U ? h->u() : h->v());
 }
 
-int fs2d_hip_stage_add_sources(fs2d_handle* h, int field, float dt) {
+static int fs2d_hip_stage_add_sources_impl(fs2d_handle* h, int field, float dt) {
     if (!h || !field_ok(field)) return fail(h, FS2D_ERR_ARGUMENT, "stage_add_sources: bad argument");
     const int src = field == FS2D_FIELD_DENSITY ? F_RS : (field == FS2D_FIELD_VELOCITY_U ? F_US : F_VS);
     if (field != FS2D_FIELD_DENSITY) h->speed_valid = false;
@@ -769,7 +844,11 @@ int fs2d_hip_stage_add_sources(fs2d_handle* h, int field, float dt) {
     return FS2D_OK;
 }
 
-int fs2d_hip_stage_diffuse(fs2d_handle* h, int field, float rate, float dt, int iterations) {
+int fs2d_hip_stage_add_sources(fs2d_handle* h, int field, float dt) {
+    return guarded(h, "fs2d_hip_stage_add_sources", [&] { return fs2d_hip_stage_add_sources_impl(h, field, dt); });
+}
+
+static int fs2d_hip_stage_diffuse_impl(fs2d_handle* h, int field, float rate, float dt, int iterations) {
     if (!h || !field_ok(field) || iterations < 0)
         return fail(h, FS2D_ERR_ARGUMENT, "stage_diffuse: bad argument");
     float*& x = field_ref(h, field);
@@ -778,30 +857,50 @@ int fs2d_hip_stage_diffuse(fs2d_handle* h, int field, float rate, float dt, int 
     return diffuse(h, x, field, rate, dt, iterations);
 }
 
-int fs2d_hip_stage_advect_density(fs2d_handle* h, float dt) {
+int fs2d_hip_stage_diffuse(fs2d_handle* h, int field, float rate, float dt, int iterations) {
+    return guarded(h, "fs2d_hip_stage_diffuse", [&] { return fs2d_hip_stage_diffuse_impl(h, field, rate, dt, iterations); });
+}
+
+static int fs2d_hip_stage_advect_density_impl(fs2d_handle* h, float dt) {
     if (!h) return fail(h, FS2D_ERR_ARGUMENT, "stage_advect_density: NULL handle");
     return advect_density(h, dt);
 }
 
-int fs2d_hip_stage_advect_velocity(fs2d_handle* h, float dt) {
+int fs2d_hip_stage_advect_density(fs2d_handle* h, float dt) {
+    return guarded(h, "fs2d_hip_stage_advect_density", [&] { return fs2d_hip_stage_advect_density_impl(h, dt); });
+}
+
+static int fs2d_hip_stage_advect_velocity_impl(fs2d_handle* h, float dt) {
     if (!h) return fail(h, FS2D_ERR_ARGUMENT, "stage_advect_velocity: NULL handle");
     return advect_velocity(h, dt);
 }
 
-int fs2d_hip_stage_project(fs2d_handle* h, int iterations) {
+int fs2d_hip_stage_advect_velocity(fs2d_handle* h, float dt) {
+    return guarded(h, "fs2d_hip_stage_advect_velocity", [&] { return fs2d_hip_stage_advect_velocity_impl(h, dt); });
+}
+
+static int fs2d_hip_stage_project_impl(fs2d_handle* h, int iterations) {
     if (!h || iterations < 0) return fail(h, FS2D_ERR_ARGUMENT, "stage_project: bad argument");
     return project(h, iterations);
 }
 
-int fs2d_hip_stage_set_boundary(fs2d_handle* h, int field) {
+int fs2d_hip_stage_project(fs2d_handle* h, int iterations) {
+    return guarded(h, "fs2d_hip_stage_project", [&] { return fs2d_hip_stage_project_impl(h, iterations); });
+}
+
+static int fs2d_hip_stage_set_boundary_impl(fs2d_handle* h, int field) {
     if (!h || !field_ok(field)) return fail(h, FS2D_ERR_ARGUMENT, "stage_set_boundary: bad argument");
     if (field != FS2D_FIELD_DENSITY) h->speed_valid = false;
     FS2D_HIP(h, launch_set_boundary(h->stream, field_ref(h, field), h->R, h->P, ring_of(field)));
     return FS2D_OK;
 }
 
+int fs2d_hip_stage_set_boundary(fs2d_handle* h, int field) {
+    return guarded(h, "fs2d_hip_stage_set_boundary", [&] { return fs2d_hip_stage_set_boundary_impl(h, field); });
+}
+
 // ---- measurement -------------------------------------------------------------
-int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
+static int fs2d_hip_time_pressure_sweeps_impl(fs2d_handle* h, int reps, float* ms) {
     if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_pressure_sweeps: bad argument");
     const Scalars s = scalars(h->R, 1.0f);
     const int K = h->prm.pressure_iterations;
@@ -819,7 +918,11 @@ int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
     return FS2D_OK;
 }
 
-int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms) {
+int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
+    return guarded(h, "fs2d_hip_time_pressure_sweeps", [&] { return fs2d_hip_time_pressure_sweeps_impl(h, reps, ms); });
+}
+
+static int fs2d_hip_time_diffuse_sweeps_impl(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms) {
     if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_diffuse_sweeps: bad argument");
     const int Kd = h->prm.diffuse_iterations;
     const int R = h->R, P = h->P;
@@ -862,7 +965,11 @@ int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float d
     return FS2D_OK;
 }
 
-int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels) {
+int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms) {
+    return guarded(h, "fs2d_hip_time_diffuse_sweeps", [&] { return fs2d_hip_time_diffuse_sweeps_impl(h, diff, visc, dt, reps, ms); });
+}
+
+static int fs2d_hip_gs_policy_impl(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels) {
     if (!h || (kind != 0 && kind != 1)) return fail(h, FS2D_ERR_ARGUMENT, "gs_policy: bad argument");
     if (h->prm.sweep_order != FS2D_ORDER_GS_LEX) return fail(h, FS2D_ERR_STATE, "gs_policy: Gauss-Seidel order only");
     const int K = kind == 0 ? h->prm.diffuse_iterations : h->prm.pressure_iterations;
@@ -879,7 +986,11 @@ int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per
     return FS2D_OK;
 }
 
-int fs2d_hip_copy_bandwidth(long long bytes, int reps, float* gbs) {
+int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels) {
+    return guarded(h, "fs2d_hip_gs_policy", [&] { return fs2d_hip_gs_policy_impl(h, kind, tile_cols, sweeps_per_unit, lag, levels); });
+}
+
+static int fs2d_hip_copy_bandwidth_impl(long long bytes, int reps, float* gbs) {
     if (bytes < 1024 || reps < 1 || !gbs) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_copy_bandwidth: bad argument");
     const size_t n4 = static_cast<size_t>(bytes) / 16;
     float4 *a = nullptr, *b = nullptr;
@@ -908,7 +1019,11 @@ int fs2d_hip_copy_bandwidth(long long bytes, int reps, float* gbs) {
     return FS2D_OK;
 }
 
-int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
+int fs2d_hip_copy_bandwidth(long long bytes, int reps, float* gbs) {
+    return guarded(nullptr, "fs2d_hip_copy_bandwidth", [&] { return fs2d_hip_copy_bandwidth_impl(bytes, reps, gbs); });
+}
+
+static int fs2d_hip_launches_per_step_impl(fs2d_handle* h, int* launches) {
     if (!h || !launches) return fail(h, FS2D_ERR_ARGUMENT, "launches_per_step: bad argument");
     const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
     auto solve_launches = [&](int K, int kind) -> int {
@@ -926,6 +1041,10 @@ int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
     return FS2D_OK;
 }
 
+int fs2d_hip_launches_per_step(fs2d_handle* h, int* launches) {
+    return guarded(h, "fs2d_hip_launches_per_step", [&] { return fs2d_hip_launches_per_step_impl(h, launches); });
+}
+
 // ---- the seeded projection (include/fs2d_cnn.h) ---------------------------
 static int residual_of(fs2d_handle* h, double* rel) {
     if (!h->res_sums) FS2D_HIP(h, hipMalloc(&h->res_sums, 2 * sizeof(double)));
@@ -938,9 +1057,13 @@ static int residual_of(fs2d_handle* h, double* rel) {
     return FS2D_OK;
 }
 
-int fs2d_hip_pressure_residual(fs2d_handle* h, double* rel) {
+static int fs2d_hip_pressure_residual_impl(fs2d_handle* h, double* rel) {
     if (!h || !rel) return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_pressure_residual: bad argument");
     return residual_of(h, rel);
+}
+
+int fs2d_hip_pressure_residual(fs2d_handle* h, double* rel) {
+    return guarded(h, "fs2d_hip_pressure_residual", [&] { return fs2d_hip_pressure_residual_impl(h, rel); });
 }
 
 // Jacobi launches on the handle's pressure field until the relative residual
@@ -1033,7 +1156,7 @@ static int mg_vcycle(fs2d_handle* h, size_t l) {
     return smooth(kMgPost);
 }
 
-int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, int max_sweeps, int check,
+static int fs2d_hip_project_seeded_impl(fs2d_handle* h, fs2d_net* net, int seed, float tol, int max_sweeps, int check,
                             int fallback, int* sweeps_run, double* rel_residual, int* fell_back) {
     if (!h || seed < 0 || seed > 3 || max_sweeps < 0 || check < 1 || !(tol >= 0.f))
         return fail(h, FS2D_ERR_ARGUMENT, "fs2d_hip_project_seeded: bad argument");
@@ -1086,6 +1209,11 @@ int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, 
     if (rel_residual) *rel_residual = rel;
     if (fell_back) *fell_back = fb;
     return FS2D_OK;
+}
+
+int fs2d_hip_project_seeded(fs2d_handle* h, fs2d_net* net, int seed, float tol, int max_sweeps, int check,
+                            int fallback, int* sweeps_run, double* rel_residual, int* fell_back) {
+    return guarded(h, "fs2d_hip_project_seeded", [&] { return fs2d_hip_project_seeded_impl(h, net, seed, tol, max_sweeps, check, fallback, sweeps_run, rel_residual, fell_back); });
 }
 
 }  // extern "C"

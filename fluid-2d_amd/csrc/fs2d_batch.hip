@@ -29,6 +29,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -566,6 +568,21 @@ int copy_sim_out(fs2d_batch* b, int sim, float* host, const float* src_arena) {
     return FS2D_OK;
 }
 
+// Exceptions thrown on the host (allocation in plan building, maps and
+// vectors) never cross the C ABI: they become an error code and message.
+template <class F>
+int guarded(fs2d_batch* h, const char* what, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return bfail(h, FS2D_ERR_HIP, std::string(what) + ": out of host memory");
+    } catch (const std::exception& e) {
+        return bfail(h, FS2D_ERR_STATE, std::string(what) + ": " + e.what());
+    } catch (...) {
+        return bfail(h, FS2D_ERR_STATE, std::string(what) + ": unexpected exception");
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -588,7 +605,7 @@ int fs2d_hip_batch_destroy(fs2d_batch* b) {
     return FS2D_OK;
 }
 
-int fs2d_hip_batch_create(int count, const fs2d_batch_sim* sims, const fs2d_params* params, fs2d_batch** out) {
+static int fs2d_hip_batch_create_impl(int count, const fs2d_batch_sim* sims, const fs2d_params* params, fs2d_batch** out) {
     if (!out) return bfail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_create: out is NULL");
     *out = nullptr;
     if (count < 1 || count > (1 << 20) || !sims)
@@ -694,7 +711,11 @@ int fs2d_hip_batch_create(int count, const fs2d_batch_sim* sims, const fs2d_para
     return FS2D_OK;
 }
 
-int fs2d_hip_batch_upload(fs2d_batch* b, int sim, const float* rho, const float* u, const float* v) {
+int fs2d_hip_batch_create(int count, const fs2d_batch_sim* sims, const fs2d_params* params, fs2d_batch** out) {
+    return guarded(nullptr, "fs2d_hip_batch_create", [&] { return fs2d_hip_batch_create_impl(count, sims, params, out); });
+}
+
+static int fs2d_hip_batch_upload_impl(fs2d_batch* b, int sim, const float* rho, const float* u, const float* v) {
     if (!b || sim < 0 || sim >= (b ? b->count : 0) || !rho || !u || !v)
         return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_upload: bad argument");
     int rc;
@@ -706,7 +727,11 @@ int fs2d_hip_batch_upload(fs2d_batch* b, int sim, const float* rho, const float*
     return FS2D_OK;
 }
 
-int fs2d_hip_batch_set_sources(fs2d_batch* b, int sim, const float* rs, const float* us, const float* vs) {
+int fs2d_hip_batch_upload(fs2d_batch* b, int sim, const float* rho, const float* u, const float* v) {
+    return guarded(b, "fs2d_hip_batch_upload", [&] { return fs2d_hip_batch_upload_impl(b, sim, rho, u, v); });
+}
+
+static int fs2d_hip_batch_set_sources_impl(fs2d_batch* b, int sim, const float* rs, const float* us, const float* vs) {
     if (!b || sim < 0 || sim >= b->count) return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_set_sources: bad argument");
     const float* src[3] = {rs, us, vs};
     const int dst[3] = {A_RS, A_US, A_VS};
@@ -726,7 +751,11 @@ int fs2d_hip_batch_set_sources(fs2d_batch* b, int sim, const float* rs, const fl
     return FS2D_OK;
 }
 
-int fs2d_hip_batch_step_n(fs2d_batch* b, int n) {
+int fs2d_hip_batch_set_sources(fs2d_batch* b, int sim, const float* rs, const float* us, const float* vs) {
+    return guarded(b, "fs2d_hip_batch_set_sources", [&] { return fs2d_hip_batch_set_sources_impl(b, sim, rs, us, vs); });
+}
+
+static int fs2d_hip_batch_step_n_impl(fs2d_batch* b, int n) {
     if (!b || n < 0) return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_step_n: bad argument");
     for (int i = 0; i < n; ++i) {
         const int rc = advance(b);
@@ -735,7 +764,11 @@ int fs2d_hip_batch_step_n(fs2d_batch* b, int n) {
     return FS2D_OK;
 }
 
-int fs2d_hip_batch_download(fs2d_batch* b, int sim, float* rho, float* u, float* v) {
+int fs2d_hip_batch_step_n(fs2d_batch* b, int n) {
+    return guarded(b, "fs2d_hip_batch_step_n", [&] { return fs2d_hip_batch_step_n_impl(b, n); });
+}
+
+static int fs2d_hip_batch_download_impl(fs2d_batch* b, int sim, float* rho, float* u, float* v) {
     if (!b || sim < 0 || sim >= b->count) return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_download: bad argument");
     int rc;
     if (rho && (rc = copy_sim_out(b, sim, rho, b->roles.a[A_RHO]))) return rc;
@@ -745,13 +778,21 @@ int fs2d_hip_batch_download(fs2d_batch* b, int sim, float* rho, float* u, float*
     return FS2D_OK;
 }
 
-int fs2d_hip_batch_sync(fs2d_batch* b) {
+int fs2d_hip_batch_download(fs2d_batch* b, int sim, float* rho, float* u, float* v) {
+    return guarded(b, "fs2d_hip_batch_download", [&] { return fs2d_hip_batch_download_impl(b, sim, rho, u, v); });
+}
+
+static int fs2d_hip_batch_sync_impl(fs2d_batch* b) {
     if (!b) return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_sync: NULL batch");
     FS2D_BHIP(b, hipStreamSynchronize(b->stream));
     return FS2D_OK;
 }
 
-int fs2d_hip_batch_info(fs2d_batch* b, int* depth, int* tile_rows, int* launches, int* graphs,
+int fs2d_hip_batch_sync(fs2d_batch* b) {
+    return guarded(b, "fs2d_hip_batch_sync", [&] { return fs2d_hip_batch_sync_impl(b); });
+}
+
+static int fs2d_hip_batch_info_impl(fs2d_batch* b, int* depth, int* tile_rows, int* launches, int* graphs,
                         long long* interior_cells) {
     if (!b) return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_info: NULL batch");
     if (depth) *depth = b->cfg.M;
@@ -765,6 +806,11 @@ int fs2d_hip_batch_info(fs2d_batch* b, int* depth, int* tile_rows, int* launches
     if (graphs) *graphs = static_cast<int>(b->graphs.size());
     if (interior_cells) *interior_cells = b->interior;
     return FS2D_OK;
+}
+
+int fs2d_hip_batch_info(fs2d_batch* b, int* depth, int* tile_rows, int* launches, int* graphs,
+                        long long* interior_cells) {
+    return guarded(b, "fs2d_hip_batch_info", [&] { return fs2d_hip_batch_info_impl(b, depth, tile_rows, launches, graphs, interior_cells); });
 }
 
 const char* fs2d_hip_batch_last_error(const fs2d_batch* b) {

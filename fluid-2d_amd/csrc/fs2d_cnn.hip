@@ -23,6 +23,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <exception>
+#include <new>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -494,6 +496,21 @@ int net_predict_fields(fs2d_net* net, hipStream_t stream, const float* d, const 
 
 const char* net_error(const fs2d_net* net) { return net ? net->err.c_str() : g_net_create_error.c_str(); }
 
+// Exceptions thrown on the host (allocation in plan building, maps and
+// vectors) never cross the C ABI: they become an error code and message.
+template <class F>
+int guarded(fs2d_net* h, const char* what, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return nfail(h, FS2D_ERR_HIP, std::string(what) + ": out of host memory");
+    } catch (const std::exception& e) {
+        return nfail(h, FS2D_ERR_STATE, std::string(what) + ": " + e.what());
+    } catch (...) {
+        return nfail(h, FS2D_ERR_STATE, std::string(what) + ": unexpected exception");
+    }
+}
+
 }  // namespace fs2d
 
 extern "C" {
@@ -517,7 +534,7 @@ int fs2d_net_destroy(fs2d_net* net) {
     return FS2D_OK;
 }
 
-int fs2d_net_create(uint64_t seed, fs2d_net** out) {
+static int fs2d_net_create_impl(uint64_t seed, fs2d_net** out) {
     if (!out) return nfail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_net_create: out is NULL");
     *out = nullptr;
     fs2d_net* net = new fs2d_net();
@@ -573,27 +590,43 @@ int fs2d_net_create(uint64_t seed, fs2d_net** out) {
     return FS2D_OK;
 }
 
-int fs2d_net_param_count(fs2d_net* net, int64_t* count) {
+int fs2d_net_create(uint64_t seed, fs2d_net** out) {
+    return guarded(nullptr, "fs2d_net_create", [&] { return fs2d_net_create_impl(seed, out); });
+}
+
+static int fs2d_net_param_count_impl(fs2d_net* net, int64_t* count) {
     if (!net || !count) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_param_count: bad argument");
     *count = net->total;
     return FS2D_OK;
 }
 
-int fs2d_net_get_params(fs2d_net* net, float* host) {
+int fs2d_net_param_count(fs2d_net* net, int64_t* count) {
+    return guarded(net, "fs2d_net_param_count", [&] { return fs2d_net_param_count_impl(net, count); });
+}
+
+static int fs2d_net_get_params_impl(fs2d_net* net, float* host) {
     if (!net || !host) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_get_params: bad argument");
     NHIP(net, hipStreamSynchronize(net->stream));
     NHIP(net, hipMemcpy(host, net->params, sizeof(float) * net->total, hipMemcpyDeviceToHost));
     return FS2D_OK;
 }
 
-int fs2d_net_get_grads(fs2d_net* net, float* host) {
+int fs2d_net_get_params(fs2d_net* net, float* host) {
+    return guarded(net, "fs2d_net_get_params", [&] { return fs2d_net_get_params_impl(net, host); });
+}
+
+static int fs2d_net_get_grads_impl(fs2d_net* net, float* host) {
     if (!net || !host) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_get_grads: bad argument");
     NHIP(net, hipStreamSynchronize(net->stream));
     NHIP(net, hipMemcpy(host, net->grads, sizeof(float) * net->total, hipMemcpyDeviceToHost));
     return FS2D_OK;
 }
 
-int fs2d_net_set_params(fs2d_net* net, const float* host) {
+int fs2d_net_get_grads(fs2d_net* net, float* host) {
+    return guarded(net, "fs2d_net_get_grads", [&] { return fs2d_net_get_grads_impl(net, host); });
+}
+
+static int fs2d_net_set_params_impl(fs2d_net* net, const float* host) {
     if (!net || !host) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_set_params: bad argument");
     NHIP(net, hipStreamSynchronize(net->stream));
     NHIP(net, hipMemcpy(net->params, host, sizeof(float) * net->total, hipMemcpyHostToDevice));
@@ -606,7 +639,11 @@ int fs2d_net_set_params(fs2d_net* net, const float* host) {
     return FS2D_OK;
 }
 
-int fs2d_net_train_step(fs2d_net* net, int n, int h, int w, const float* d, const float* prev, const float* target,
+int fs2d_net_set_params(fs2d_net* net, const float* host) {
+    return guarded(net, "fs2d_net_set_params", [&] { return fs2d_net_set_params_impl(net, host); });
+}
+
+static int fs2d_net_train_step_impl(fs2d_net* net, int n, int h, int w, const float* d, const float* prev, const float* target,
                         float lr, float* loss) {
     int rc;
     if ((rc = check_shape(net, n, h, w))) return rc;
@@ -645,7 +682,12 @@ int fs2d_net_train_step(fs2d_net* net, int n, int h, int w, const float* d, cons
     return FS2D_OK;
 }
 
-int fs2d_net_predict_host(fs2d_net* net, int n, int h, int w, const float* d, const float* prev, float* out) {
+int fs2d_net_train_step(fs2d_net* net, int n, int h, int w, const float* d, const float* prev, const float* target,
+                        float lr, float* loss) {
+    return guarded(net, "fs2d_net_train_step", [&] { return fs2d_net_train_step_impl(net, n, h, w, d, prev, target, lr, loss); });
+}
+
+static int fs2d_net_predict_host_impl(fs2d_net* net, int n, int h, int w, const float* d, const float* prev, float* out) {
     int rc;
     if ((rc = check_shape(net, n, h, w))) return rc;
     if (!d || !prev || !out) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_predict_host: NULL argument");
@@ -665,7 +707,11 @@ int fs2d_net_predict_host(fs2d_net* net, int n, int h, int w, const float* d, co
     return FS2D_OK;
 }
 
-int fs2d_net_time_forward(fs2d_net* net, int n, int h, int w, int reps, float* ms) {
+int fs2d_net_predict_host(fs2d_net* net, int n, int h, int w, const float* d, const float* prev, float* out) {
+    return guarded(net, "fs2d_net_predict_host", [&] { return fs2d_net_predict_host_impl(net, n, h, w, d, prev, out); });
+}
+
+static int fs2d_net_time_forward_impl(fs2d_net* net, int n, int h, int w, int reps, float* ms) {
     int rc;
     if ((rc = check_shape(net, n, h, w))) return rc;
     if (reps < 1 || !ms) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_time_forward: bad argument");
@@ -688,7 +734,11 @@ int fs2d_net_time_forward(fs2d_net* net, int n, int h, int w, int reps, float* m
     return FS2D_OK;
 }
 
-int fs2d_net_forward_flops(fs2d_net* net, int n, int h, int w, double* flops) {
+int fs2d_net_time_forward(fs2d_net* net, int n, int h, int w, int reps, float* ms) {
+    return guarded(net, "fs2d_net_time_forward", [&] { return fs2d_net_time_forward_impl(net, n, h, w, reps, ms); });
+}
+
+static int fs2d_net_forward_flops_impl(fs2d_net* net, int n, int h, int w, double* flops) {
     int rc;
     if ((rc = check_shape(net, n, h, w))) return rc;
     if (!flops) return nfail(net, FS2D_ERR_ARGUMENT, "fs2d_net_forward_flops: bad argument");
@@ -700,6 +750,10 @@ int fs2d_net_forward_flops(fs2d_net* net, int n, int h, int w, double* flops) {
         f += 2.0 * 9.0 * kLayers[l].cin * kLayers[l].cout * p1 / static_cast<double>(1 << (2 * level[l]));
     *flops = f;
     return FS2D_OK;
+}
+
+int fs2d_net_forward_flops(fs2d_net* net, int n, int h, int w, double* flops) {
+    return guarded(net, "fs2d_net_forward_flops", [&] { return fs2d_net_forward_flops_impl(net, n, h, w, flops); });
 }
 
 void* fs2d_net_stream(fs2d_net* net) { return net ? static_cast<void*>(net->stream) : nullptr; }

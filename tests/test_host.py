@@ -55,6 +55,8 @@ def test_default_params_are_the_reference_literals():
     (10, 10, {"gs_lag": 2}, "gs_lag"),
     (10, 10, {"jacobi_kernel": 3}, "jacobi_kernel"),
     (10, 10, {"launch_mode": 2}, "launch_mode"),
+    (4098, 4098, {"pressure_iterations": 1000000}, "too many Gauss-Seidel sweeps"),
+    (32768, 32768, {"diffuse_iterations": 20000, "gs_block": 1}, "too many Gauss-Seidel sweeps"),
 ])
 def test_create_validates_arguments_before_touching_the_device(rows, cols, kw, msg):
     with pytest.raises(fs2d_amd.Fs2dError, match=msg):
