@@ -130,17 +130,6 @@ __global__ __launch_bounds__(64) void kb_fold_speed(unsigned* __restrict__ max_b
     if (t == 0) max_bits[s] = m;
 }
 
-__global__ __launch_bounds__(CBX* CBY) void kb_scatter_density(const BatchSim* __restrict__ sims,
-                                                               const int2* __restrict__ work, Roles r,
-                                                               const unsigned* __restrict__ max_bits) {
-    int s, i, j, blk;
-    if (!cell_of(sims, work, gridDim.x, s, i, j, blk)) return;
-    const BatchSim& S = sims[s];
-    const size_t o = S.off;
-    cells::scatter_density(r.a[A_T1] + o, r.a[A_RHO] + o, r.a[A_U] + o, r.a[A_V] + o, S.R, S.P, S.dt0,
-                           __uint_as_float(max_bits[s]), i, j);
-}
-
 __global__ __launch_bounds__(CBX* CBY) void kb_gather_velocity(const BatchSim* __restrict__ sims,
                                                                const int2* __restrict__ work, Roles r) {
     int s, i, j, blk;
@@ -299,6 +288,11 @@ struct fs2d_batch {
     int cell_blocks = 0;
     unsigned* max_bits = nullptr;  // per simulation
     unsigned* slots = nullptr;     // per simulation kSimSlots * kSimSlotStride words, zero between uses
+    ScatterSim* scatter_sims = nullptr;  // the density scatter's descriptors (fs2d_scatter.hip)
+    int2* scatter_windows = nullptr;     // its window path's work list, one entry per wave
+    void* scatter_scratch = nullptr;     // its sorted path's buffers
+    int* scatter_flag = nullptr;         // its sorted-path gate
+    BatchScatter scatter{};
     JacobiConfig cfg;
     bool tuned = false;
     bool speed_valid = false;
@@ -484,8 +478,8 @@ int step(fs2d_batch* b) {
     FS2D_BHIP(b, cell_launch_check());
     if (!b->tuned && (rc = tune(b))) return rc;
     if ((rc = solve(b, SWEEP_DIFFUSE))) return rc;
-    hipLaunchKernelGGL(kb_scatter_density, grid, blk, 0, b->stream, b->sims, b->cell_work, b->roles, b->max_bits);
-    FS2D_BHIP(b, cell_launch_check());
+    FS2D_BHIP(b, launch_batch_scatter(b->stream, b->scatter, b->roles.a[A_T1], b->roles.a[A_RHO], b->roles.a[A_U],
+                                      b->roles.a[A_V], b->max_bits));
     std::swap(b->roles.a[A_RHO], b->roles.a[A_T1]);
     std::swap(b->roles.a[A_U], b->roles.a[A_UN]);
     std::swap(b->roles.a[A_V], b->roles.a[A_VN]);
@@ -598,6 +592,10 @@ int fs2d_hip_batch_destroy(fs2d_batch* b) {
     if (b->cell_work) (void)hipFree(b->cell_work);
     if (b->max_bits) (void)hipFree(b->max_bits);
     if (b->slots) (void)hipFree(b->slots);
+    if (b->scatter_sims) (void)hipFree(b->scatter_sims);
+    if (b->scatter_windows) (void)hipFree(b->scatter_windows);
+    if (b->scatter_scratch) (void)hipFree(b->scatter_scratch);
+    if (b->scatter_flag) (void)hipFree(b->scatter_flag);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -641,7 +639,9 @@ static int fs2d_hip_batch_create_impl(int count, const fs2d_batch_sim* sims, con
     b->spec.assign(sims, sims + count);
     b->host_sims.resize(count);
     size_t off = 0;
-    std::vector<int2> cw;
+    std::vector<int2> cw, sw;  // single-pass stage blocks; the scatter's window-path waves
+    std::vector<ScatterSim> ss(count);
+    long long scatter_cells = 0, scatter_targets = 0;
     for (int s = 0; s < count; ++s) {
         BatchSim& S = b->host_sims[s];
         const fs2d_batch_sim& q = sims[s];
@@ -669,6 +669,13 @@ static int fs2d_hip_batch_create_impl(int count, const fs2d_batch_sim* sims, con
         S.cell_blocks = gx * gy;
         for (int by = 0; by < gy; ++by)
             for (int bx = 0; bx < gx; ++bx) cw.push_back(make_int2(s, (by << 16) | bx));
+        ss[s] = ScatterSim{S.off, S.R, S.P, S.dt0, static_cast<unsigned>(scatter_cells),
+                           static_cast<unsigned>(scatter_targets)};
+        scatter_cells += static_cast<long long>(n) * n;
+        scatter_targets += static_cast<long long>(S.R - 1) * (S.R - 1);
+        for (int band = 0; band < (n + kBatchBand - 1) / kBatchBand; ++band)
+            for (int strip = 0; strip < (n + kBatchStrip - 1) / kBatchStrip; ++strip)
+                sw.push_back(make_int2(s, (band << 16) | strip));
         b->interior += static_cast<long long>(n) * n;
         b->order.push_back(s);
     }
@@ -700,6 +707,24 @@ static int fs2d_hip_batch_create_impl(int count, const fs2d_batch_sim* sims, con
     if ((e = hipMalloc(&b->slots, sizeof(unsigned) * slot_words)) != hipSuccess) return bail(e, "hipMalloc(slots)");
     hipLaunchKernelGGL(kb_fill, dim3((unsigned)std::min<size_t>((slot_words + 255) / 256, 4096)), dim3(256), 0,
                        b->stream, b->slots, slot_words);
+    // the density scatter: descriptors, window work list, sorted-path buffers
+    if (scatter_cells >= (1ll << 31) || scatter_targets >= (1ll << 32) - 1)
+        return bail(hipErrorInvalidValue, "too many cells for the batched density scatter");
+    if ((e = hipMalloc(&b->scatter_sims, sizeof(ScatterSim) * count)) != hipSuccess)
+        return bail(e, "hipMalloc(scatter descriptors)");
+    if ((e = hipMalloc(&b->scatter_windows, sizeof(int2) * sw.size())) != hipSuccess)
+        return bail(e, "hipMalloc(scatter work)");
+    if ((e = hipMalloc(&b->scatter_scratch, batch_scatter_bytes(scatter_cells, scatter_targets))) != hipSuccess)
+        return bail(e, "hipMalloc(scatter buffers)");
+    if ((e = hipMalloc(&b->scatter_flag, 256)) != hipSuccess) return bail(e, "hipMalloc(scatter gate)");
+    if ((e = hipMemcpyAsync(b->scatter_sims, ss.data(), sizeof(ScatterSim) * count, hipMemcpyHostToDevice,
+                            b->stream)) != hipSuccess)
+        return bail(e, "hipMemcpy(scatter descriptors)");
+    if ((e = hipMemcpyAsync(b->scatter_windows, sw.data(), sizeof(int2) * sw.size(), hipMemcpyHostToDevice,
+                            b->stream)) != hipSuccess)
+        return bail(e, "hipMemcpy(scatter work)");
+    b->scatter = BatchScatter{b->scatter_sims, count, b->scatter_windows, static_cast<int>(sw.size()), scatter_cells,
+                              scatter_targets, b->scatter_scratch, b->scatter_flag};
     if ((e = hipMemcpyAsync(b->cell_work, cw.data(), sizeof(int2) * cw.size(), hipMemcpyHostToDevice, b->stream)) !=
         hipSuccess)
         return bail(e, "hipMemcpy(work)");
@@ -798,10 +823,10 @@ static int fs2d_hip_batch_info_impl(fs2d_batch* b, int* depth, int* tile_rows, i
     if (depth) *depth = b->cfg.M;
     if (tile_rows) *tile_rows = b->cfg.TY;
     if (launches) {
-        // max speed (first step only) aside: add, diffuse launches, scatter,
-        // 2 x (divergence, pressure launches, gradient, fold), gather
+        // max speed (first step only) aside: add, diffuse launches, the scatter's
+        // kernels, 2 x (divergence, pressure launches, gradient, fold), gather
         const int d = static_cast<int>(b->cfg.diffuse_count.size()), p = static_cast<int>(b->cfg.pressure_count.size());
-        *launches = b->tuned ? 1 + d + 1 + 2 * (3 + p) + 1 : 0;
+        *launches = b->tuned ? 1 + d + batch_scatter_launches(b->scatter.targets) + 2 * (3 + p) + 1 : 0;
     }
     if (graphs) *graphs = static_cast<int>(b->graphs.size());
     if (interior_cells) *interior_cells = b->interior;

@@ -73,52 +73,6 @@ __device__ __forceinline__ float fold_speed(float m, float a, float b) {
     return m;
 }
 
-// Density advection, src/fluid_solver_cpu.cpp:127-152, as a gather.
-//
-// The reference scatters each source cell (i, j) bilinearly onto the four
-// cells around (i + dt0*v, j + dt0*u) with `+=`, in row-major source order,
-// into a zero-filled grid.  Here each destination cell walks the source cells
-// that can reach it, in the same row-major order, and sums the same products
-// in the same order starting from +0, so every destination gets the
-// reference's exact float sum with no atomics.  The window is bounded by the
-// largest displacement dt0 * max|vel| (a source moves at most that far, and
-// lands on trunc(pos) and trunc(pos) + 1).  A non-finite source position is
-// skipped (the reference converts NaN to size_t there: undefined behaviour).
-__device__ __forceinline__ void scatter_density(float* __restrict__ out, const float* __restrict__ rho0,
-                                                const float* __restrict__ u, const float* __restrict__ v, int R,
-                                                int P, float dt0, float max_vel, int di, int dj) {
-    // A source at row si lands on rows trunc(y), trunc(y) + 1 with
-    // |y - si| <= |fl(dt0 v)| + ulp(y) / 2 <= dt0 * max|v| + 2^-8 (y < 2^15),
-    // so it reaches row di only if |si - di| <= W = floor(that bound) + 1.
-    const float reach = dt0 * max_vel + 0x1p-8f;
-    const int r = (reach < (float)R) ? (int)reach + 1 : R;
-    const int i_lo = max(1, di - r), i_hi = min(R - 2, di + r);
-    const int j_lo = max(1, dj - r), j_hi = min(R - 2, dj + r);
-    const float hi = (float)R - 1.5f;
-    float acc = 0.f;
-    for (int si = i_lo; si <= i_hi; ++si) {
-        const size_t row = (size_t)si * P;
-        const float fi = (float)si;
-        for (int sj = j_lo; sj <= j_hi; ++sj) {
-            const float x = (float)sj + dt0 * u[row + sj];
-            const float y = fi + dt0 * v[row + sj];
-            if (x < 0.5f || x > hi || y < 0.5f || y > hi) continue;
-            if (x != x || y != y) continue;
-            const int j0 = (int)x, i0 = (int)y;
-            const int ddi = di - i0, ddj = dj - j0;
-            if ((unsigned)ddi > 1u || (unsigned)ddj > 1u) continue;
-            const float s0 = x - (float)j0;
-            const float s1 = 1.0f - s0;
-            const float s2 = y - (float)i0;
-            const float s3 = 1.0f - s2;
-            const float w = (ddj == 0) ? ((ddi == 0) ? s1 * s3 : s1 * s2) : ((ddi == 0) ? s0 * s3 : s0 * s2);
-            acc = acc + w * rho0[row + sj];
-        }
-    }
-    out[(size_t)di * P + dj] = acc;
-    write_ring(out, di, dj, acc, R, P, Ring{false, false});
-}
-
 // std::min / std::max as <algorithm> defines them (the clamp's NaN handling
 // depends on the operand order, src/fluid_solver_cpu.cpp:158-159).
 __device__ __forceinline__ float std_min(float a, float b) { return (b < a) ? b : a; }

@@ -94,6 +94,32 @@ hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, 
                                   int P, float dt0, const unsigned* max_bits, void* sort_scratch);
 size_t scatter_sort_bytes(int R);
 int scatter_launches(int R);  // kernels launch_scatter_density enqueues with sort_scratch
+
+// The density scatter of a batch of simulations (fs2d_batch.hip) on the
+// kernels of fs2d_scatter.hip.  Simulation s's fields start `off` floats into
+// the arenas; its interior cells are numbered from `cells` and its target
+// cells ((R - 1)^2 of them) from `targets` across the batch.
+struct ScatterSim {
+    size_t off;
+    int R, P;
+    float dt0;
+    unsigned cells, targets;
+};
+constexpr int kBatchStrip = 56, kBatchBand = 32;  // window-path strip columns and band rows in a batch
+struct BatchScatter {
+    const ScatterSim* sims;  // device, `count` entries
+    int count;
+    const int2* windows;     // device: (simulation, band << 16 | strip), one per wave of the window path
+    int window_waves;
+    long long cells, targets;  // totals over the batch
+    void* scratch;           // batch_scatter_bytes(cells, targets)
+    int* flag;               // device word, set every step: some simulation takes the sorted path
+};
+size_t batch_scatter_bytes(long long cells, long long targets);
+int batch_scatter_launches(long long targets);
+// max_bits[s]: the largest finite |u|, |v| of simulation s.
+hipError_t launch_batch_scatter(hipStream_t s, const BatchScatter& B, float* out, const float* rho0, const float* u,
+                                const float* v, const unsigned* max_bits);
 hipError_t launch_gather_velocity(hipStream_t s, float* u1, float* v1, const float* u0, const float* v0,
                                   int R, int P, float dt0);
 // p_zero (may be null): also zero this pressure iterate, ring included.

@@ -297,6 +297,20 @@ int window_slots() {
 }
 
 // ---- sorted path ---------------------------------------------------------------
+// The sorted path's kernels are launched every step and leave at once unless
+// the gate says some source may move past the window path's radius: on one
+// grid from its largest speed, in a batch from a flag set per step.
+struct GridGate {
+    const unsigned* max_bits;
+    float dt0;
+    int R;
+    __device__ bool sorted() const { return scatter_radius(max_bits, dt0, R) > kWindow; }
+};
+struct FlagGate {
+    const int* flag;
+    __device__ bool sorted() const { return *flag != 0; }
+};
+
 // Items are 64-bit: target cell i0 * (R - 1) + j0 in the high word (or
 // (R - 1)^2 for a skipped source, sorted last), the source's interior
 // row-major index in the low word.  The sort takes 9-bit digits over chunks of
@@ -341,10 +355,10 @@ __global__ __launch_bounds__(NT) void k_sort_keys(unsigned long long* __restrict
 }
 
 // Chunk counts of digit (key >> shift) & (ND - 1): hist[digit * chunks + chunk].
+template <class G>
 __global__ __launch_bounds__(NT) void k_sort_hist(const unsigned long long* __restrict__ items, long long N, int shift,
-                                                  unsigned* __restrict__ hist, int chunks, int R, float dt0,
-                                                  const unsigned* __restrict__ max_bits) {
-    if (scatter_radius(max_bits, dt0, R) <= kWindow) return;
+                                                  unsigned* __restrict__ hist, int chunks, G gate) {
+    if (!gate.sorted()) return;
     __shared__ unsigned cnt[ND];
     for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
         for (int d = threadIdx.x; d < ND; d += NT) cnt[d] = 0u;
@@ -360,9 +374,9 @@ __global__ __launch_bounds__(NT) void k_sort_hist(const unsigned long long* __re
 }
 
 // Exclusive scan of hist (digit-major) in place, one block.
-__global__ __launch_bounds__(1024) void k_sort_scan(unsigned* __restrict__ hist, int total, int R, float dt0,
-                                                    const unsigned* __restrict__ max_bits) {
-    if (scatter_radius(max_bits, dt0, R) <= kWindow) return;
+template <class G>
+__global__ __launch_bounds__(1024) void k_sort_scan(unsigned* __restrict__ hist, int total, G gate) {
+    if (!gate.sorted()) return;
     __shared__ unsigned part[1024];
     const int t = threadIdx.x;
     const int per = (total + 1023) / 1024;
@@ -387,11 +401,11 @@ __global__ __launch_bounds__(1024) void k_sort_scan(unsigned* __restrict__ hist,
 
 // Stable scatter of each chunk by digit: items of a chunk keep their order
 // within each digit (ranks by round, wave, then lane).
+template <class G>
 __global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned long long* __restrict__ in,
                                                      unsigned long long* __restrict__ out, long long N, int shift,
-                                                     const unsigned* __restrict__ offs, int chunks, int R, float dt0,
-                                                     const unsigned* __restrict__ max_bits) {
-    if (scatter_radius(max_bits, dt0, R) <= kWindow) return;
+                                                     const unsigned* __restrict__ offs, int chunks, G gate) {
+    if (!gate.sorted()) return;
     __shared__ unsigned run[ND];           // items of each digit placed by earlier rounds
     __shared__ unsigned wcnt[NT / 64][ND];  // items of each digit in each wave of this round
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
@@ -432,11 +446,10 @@ __global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned long long* _
 }
 
 // For each target cell, the range [first, last) of its items after the sort.
+template <class G>
 __global__ __launch_bounds__(NT) void k_sort_bounds(const unsigned long long* __restrict__ items, long long N,
-                                                    int2* __restrict__ bounds, int R, float dt0,
-                                                    const unsigned* __restrict__ max_bits) {
-    if (scatter_radius(max_bits, dt0, R) <= kWindow) return;
-    const unsigned nb = (unsigned)(R - 1) * (unsigned)(R - 1);
+                                                    int2* __restrict__ bounds, unsigned nb, G gate) {
+    if (!gate.sorted()) return;
     for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < N; i += (long long)gridDim.x * NT) {
         const unsigned key = (unsigned)(items[i] >> 32);
         if (key >= nb) continue;
@@ -494,15 +507,163 @@ __global__ __launch_bounds__(NT) void k_scatter_merge(float* __restrict__ out, c
     }
 }
 
+// ---- a batch of simulations -------------------------------------------------------
+// Window path: one wave per (simulation, band, strip) of a fixed work list,
+// strips of kBatchStrip destination columns with kWindow source columns
+// either side (so every radius up to kWindow fits); a wave whose simulation
+// moves farther leaves, and the batched sorted path serves that simulation.
+__global__ __launch_bounds__(NT, 6) void k_bscatter_window(float* __restrict__ out, const float* __restrict__ rho0,
+                                                           const float* __restrict__ u, const float* __restrict__ v,
+                                                           const ScatterSim* __restrict__ sims,
+                                                           const int2* __restrict__ windows, int waves,
+                                                           const unsigned* __restrict__ max_bits) {
+    const int item = (int)blockIdx.x * kStripWaves + (int)(threadIdx.x >> 6);
+    if (item >= waves) return;  // whole waves leave; no block-wide barrier follows
+    const int2 wk = windows[item];
+    const ScatterSim S = sims[wk.x];
+    const int W = scatter_radius(max_bits + wk.x, S.dt0, S.R);
+    if (W > kWindow) return;
+    const int lane = threadIdx.x & 63;
+    const int band = wk.y >> 16, strip = wk.y & 0xffff;
+    const int col = 1 + strip * kBatchStrip - kWindow + lane;
+    const bool dest = lane >= kWindow && lane < kWindow + kBatchStrip && col <= S.R - 2;
+    const int r0 = 1 + kBatchBand * band, r1 = min(r0 + kBatchBand, S.R - 1);
+    out += S.off;
+    rho0 += S.off;
+    u += S.off;
+    v += S.off;
+    switch (W) {
+        case 1: window_band_r1(out, rho0, u, v, S.R, S.P, S.dt0, col, r0, r1, dest); break;
+        case 2: window_band<2>(out, rho0, u, v, S.R, S.P, S.dt0, col, r0, r1, dest); break;
+        case 3: window_band<3>(out, rho0, u, v, S.R, S.P, S.dt0, col, r0, r1, dest); break;
+        default: window_band<4>(out, rho0, u, v, S.R, S.P, S.dt0, col, r0, r1, dest); break;
+    }
+}
+
+// The batch's gate: some simulation moves farther than the window path serves.
+__global__ __launch_bounds__(256) void k_bsort_gate(int* __restrict__ flag, const ScatterSim* __restrict__ sims,
+                                                    int count, const unsigned* __restrict__ max_bits) {
+    int any = 0;
+    for (int s = threadIdx.x; s < count; s += 256) any |= scatter_radius(max_bits + s, sims[s].dt0, sims[s].R) > kWindow;
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) *flag = any;
+}
+
+// Simulation of batch cell g: the last one whose first cell is <= g.
+__device__ __forceinline__ int sim_of_cell(const ScatterSim* __restrict__ sims, int count, unsigned g) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sims[mid].cells <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Sort items over the whole batch: the sources of simulations on the sorted
+// path keyed by their batch target cell, every other source keyed past the
+// last target (sorted last, never merged).
+__global__ __launch_bounds__(NT) void k_bsort_keys(unsigned long long* __restrict__ items, unsigned* __restrict__ hist,
+                                                   int2* __restrict__ bounds, const float* __restrict__ u,
+                                                   const float* __restrict__ v, const ScatterSim* __restrict__ sims,
+                                                   int count, long long N, unsigned nb, int chunks,
+                                                   const unsigned* __restrict__ max_bits, FlagGate gate) {
+    if (!gate.sorted()) return;
+    __shared__ unsigned cnt[ND];
+    for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < (long long)nb; i += (long long)gridDim.x * NT)
+        bounds[i] = make_int2(0, 0);
+    for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
+        for (int d = threadIdx.x; d < ND; d += NT) cnt[d] = 0u;
+        __syncthreads();
+        for (int q = 0; q < IPT; ++q) {
+            const long long g = (long long)c * CH + q * NT + threadIdx.x;
+            if (g >= N) break;
+            const int s = sim_of_cell(sims, count, (unsigned)g);
+            const ScatterSim S = sims[s];
+            const int n = S.R - 2;
+            const int local = (int)((unsigned)g - S.cells);
+            const int si = 1 + local / n, sj = 1 + local % n;
+            unsigned key = nb;
+            if (scatter_radius(max_bits + s, S.dt0, S.R) > kWindow) {
+                const size_t o = S.off + (size_t)si * S.P + sj;
+                int i0, j0;
+                float s0, s2;
+                if (source_lands(si, sj, u[o], v[o], S.dt0, (float)S.R - 1.5f, i0, j0, s0, s2))
+                    key = S.targets + (unsigned)i0 * (unsigned)(S.R - 1) + (unsigned)j0;
+            }
+            items[g] = ((unsigned long long)key << 32) | (unsigned long long)g;
+            atomicAdd(&cnt[key & (ND - 1)], 1u);
+        }
+        __syncthreads();
+        for (int d = threadIdx.x; d < ND; d += NT) hist[d * chunks + c] = cnt[d];
+        __syncthreads();
+    }
+}
+
+// As k_scatter_merge, for the cells of simulations on the sorted path.
+__global__ __launch_bounds__(NT) void k_bscatter_merge(float* __restrict__ out, const float* __restrict__ rho0,
+                                                       const float* __restrict__ u, const float* __restrict__ v,
+                                                       const unsigned long long* __restrict__ items,
+                                                       const int2* __restrict__ bounds,
+                                                       const ScatterSim* __restrict__ sims, int count, long long N,
+                                                       const unsigned* __restrict__ max_bits, FlagGate gate) {
+    if (!gate.sorted()) return;
+    for (long long g = (long long)blockIdx.x * NT + threadIdx.x; g < N; g += (long long)gridDim.x * NT) {
+        const int s = sim_of_cell(sims, count, (unsigned)g);
+        const ScatterSim S = sims[s];
+        if (scatter_radius(max_bits + s, S.dt0, S.R) <= kWindow) continue;
+        const int n = S.R - 2, R = S.R, P = S.P;
+        const float hi = (float)R - 1.5f;
+        const int local = (int)((unsigned)g - S.cells);
+        const int di = 1 + local / n, dj = 1 + local % n;
+        int cur[4], end[4];
+        const int ti[4] = {di, di - 1, di, di - 1}, tj[4] = {dj, dj, dj - 1, dj - 1};
+        for (int k = 0; k < 4; ++k) {
+            const int2 b = bounds[(size_t)S.targets + (size_t)ti[k] * (R - 1) + tj[k]];
+            cur[k] = b.x;
+            end[k] = b.y;
+        }
+        float acc = 0.f;
+        for (;;) {
+            int best = -1;
+            unsigned bs = 0xffffffffu;
+            for (int k = 0; k < 4; ++k) {
+                if (cur[k] >= end[k]) continue;
+                const unsigned src = (unsigned)items[cur[k]];
+                if (src < bs) {
+                    bs = src;
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            ++cur[best];
+            const int sl = (int)(bs - S.cells);
+            const int si = 1 + sl / n, sj = 1 + sl % n;
+            const size_t o = S.off + (size_t)si * P + sj;
+            int i0, j0;
+            float s0, s2;
+            source_lands(si, sj, u[o], v[o], S.dt0, hi, i0, j0, s0, s2);
+            const float s1 = 1.0f - s0, s3 = 1.0f - s2;
+            const float r0 = rho0[o];
+            const float sh = (best == 0) ? s1 * s3 * r0 : (best == 1) ? s1 * s2 * r0 : (best == 2) ? s0 * s3 * r0 : s0 * s2 * r0;
+            acc = acc + sh;
+        }
+        float* o = out + S.off;
+        o[(size_t)di * P + dj] = acc;
+        cells::write_ring(o, di, dj, acc, R, P, Ring{false, false});
+    }
+}
+
 }  // namespace
 
-// Bits of the sort keys: targets 0 .. (R - 1)^2 (the last one for skipped sources).
-static int sort_key_bits(int R) {
-    const unsigned long long nb = (unsigned long long)(R - 1) * (unsigned long long)(R - 1);
+// Bits of sort keys 0 .. nb (nb itself for skipped sources).
+static int key_bits(unsigned long long nb) {
     int bits = 1;
     while ((1ull << bits) <= nb) ++bits;
     return bits;
 }
+// Targets 0 .. (R - 1)^2 of one grid.
+static int sort_key_bits(int R) { return key_bits((unsigned long long)(R - 1) * (unsigned long long)(R - 1)); }
 
 int scatter_launches(int R) {
     const int passes = (sort_key_bits(R) + DB - 1) / DB;
@@ -540,20 +701,72 @@ hipError_t launch_scatter_density(hipStream_t s, float* out, const float* rho0, 
     hipLaunchKernelGGL(k_sort_keys, dim3(chunk_blocks), dim3(NT), 0, s, items0, hist, bounds, u, v, R, P, dt0, chunks,
                        max_bits);
     const int bits = sort_key_bits(R);
+    const GridGate gate{max_bits, dt0, R};
     unsigned long long* src = items0;
     unsigned long long* dst = items1;
     for (int shift = 0; shift < bits; shift += DB) {
         if (shift > 0)
-            hipLaunchKernelGGL(k_sort_hist, dim3(chunk_blocks), dim3(NT), 0, s, src, N, shift, hist, chunks, R, dt0,
-                               max_bits);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, hist, ND * chunks, R, dt0, max_bits);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(chunk_blocks), dim3(NT), 0, s, src, dst, N, shift, hist, chunks, R, dt0,
-                           max_bits);
+            hipLaunchKernelGGL(k_sort_hist<GridGate>, dim3(chunk_blocks), dim3(NT), 0, s, src, N, shift, hist, chunks,
+                               gate);
+        hipLaunchKernelGGL(k_sort_scan<GridGate>, dim3(1), dim3(1024), 0, s, hist, ND * chunks, gate);
+        hipLaunchKernelGGL(k_sort_scatter<GridGate>, dim3(chunk_blocks), dim3(NT), 0, s, src, dst, N, shift, hist,
+                           chunks, gate);
         std::swap(src, dst);
     }
-    hipLaunchKernelGGL(k_sort_bounds, dim3(cell_blocks), dim3(NT), 0, s, src, N, bounds, R, dt0, max_bits);
+    hipLaunchKernelGGL(k_sort_bounds<GridGate>, dim3(cell_blocks), dim3(NT), 0, s, src, N, bounds, (unsigned)nb, gate);
     hipLaunchKernelGGL(k_scatter_merge, dim3(cell_blocks), dim3(NT), 0, s, out, rho0, u, v, src, bounds, R, P, dt0,
                        max_bits);
+    return hipGetLastError();
+}
+
+size_t batch_scatter_bytes(long long cells, long long targets) {
+    const size_t N = static_cast<size_t>(cells);
+    const size_t chunks = std::max<size_t>((N + CH - 1) / CH, 1);
+    return 2 * N * sizeof(unsigned long long) + static_cast<size_t>(targets) * sizeof(int2) +
+           ND * chunks * sizeof(unsigned) + 4096;
+}
+
+int batch_scatter_launches(long long targets) {
+    const int passes = (key_bits(static_cast<unsigned long long>(targets)) + DB - 1) / DB;
+    return 4 + 3 * passes;  // window, gate, keys, (passes - 1) histograms, passes x (scan, scatter), bounds, merge
+}
+
+hipError_t launch_batch_scatter(hipStream_t s, const BatchScatter& B, float* out, const float* rho0, const float* u,
+                                const float* v, const unsigned* max_bits) {
+    if (B.cells <= 0 || B.cells >= (1ll << 31) || B.targets >= (1ll << 32) - 1 || !B.scratch || !B.flag)
+        return hipErrorInvalidValue;
+    if (B.window_waves > 0)
+        hipLaunchKernelGGL(k_bscatter_window, dim3((B.window_waves + kStripWaves - 1) / kStripWaves), dim3(NT), 0, s,
+                           out, rho0, u, v, B.sims, B.windows, B.window_waves, max_bits);
+    hipLaunchKernelGGL(k_bsort_gate, dim3(1), dim3(256), 0, s, B.flag, B.sims, B.count, max_bits);
+    // sorted path (returns at once unless the gate is set)
+    const long long N = B.cells;
+    const int chunks = static_cast<int>(std::max<long long>((N + CH - 1) / CH, 1));
+    auto* items0 = static_cast<unsigned long long*>(B.scratch);
+    auto* items1 = items0 + N;
+    auto* bounds = reinterpret_cast<int2*>(items1 + N);
+    auto* hist = reinterpret_cast<unsigned*>(bounds + B.targets);
+    const unsigned nb = static_cast<unsigned>(B.targets);
+    const unsigned chunk_blocks = static_cast<unsigned>(std::min(chunks, kSortBlocks));
+    const unsigned cell_blocks = static_cast<unsigned>(std::min<long long>((N + NT - 1) / NT, kSortBlocks));
+    const FlagGate gate{B.flag};
+    hipLaunchKernelGGL(k_bsort_keys, dim3(chunk_blocks), dim3(NT), 0, s, items0, hist, bounds, u, v, B.sims, B.count,
+                       N, nb, chunks, max_bits, gate);
+    const int bits = key_bits(nb);
+    unsigned long long* src = items0;
+    unsigned long long* dst = items1;
+    for (int shift = 0; shift < bits; shift += DB) {
+        if (shift > 0)
+            hipLaunchKernelGGL(k_sort_hist<FlagGate>, dim3(chunk_blocks), dim3(NT), 0, s, src, N, shift, hist, chunks,
+                               gate);
+        hipLaunchKernelGGL(k_sort_scan<FlagGate>, dim3(1), dim3(1024), 0, s, hist, ND * chunks, gate);
+        hipLaunchKernelGGL(k_sort_scatter<FlagGate>, dim3(chunk_blocks), dim3(NT), 0, s, src, dst, N, shift, hist,
+                           chunks, gate);
+        std::swap(src, dst);
+    }
+    hipLaunchKernelGGL(k_sort_bounds<FlagGate>, dim3(cell_blocks), dim3(NT), 0, s, src, N, bounds, nb, gate);
+    hipLaunchKernelGGL(k_bscatter_merge, dim3(cell_blocks), dim3(NT), 0, s, out, rho0, u, v, src, bounds, B.sims,
+                       B.count, N, max_bits, gate);
     return hipGetLastError();
 }
 

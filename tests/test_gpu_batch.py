@@ -29,8 +29,8 @@ def start(k, R):
     return st, src
 
 
-def run_and_check(sims, block, mode, steps, oracle_lib):
-    starts = [start(k, s[0]) for k, s in enumerate(sims)]
+def run_and_check(sims, block, mode, steps, oracle_lib, starts=None):
+    starts = starts or [start(k, s[0]) for k, s in enumerate(sims)]
     want = [[x.copy() for x in st] for st, _ in starts]
     z = [np.zeros((s[0], s[0]), np.float32) for s in sims]
     with fs2d_amd.Batch(sims, jacobi_block=block, launch_mode=mode) as b:
@@ -97,3 +97,48 @@ def test_batch_single_simulation_equals_handle():
         gb, gs = b.download(0), s.download()
     for f in range(3):
         assert bits_equal(gb[f], gs[f]), describe(gb[f], gs[f])
+
+
+def scaled_start(R, dt, reach, seed):
+    """A random state whose largest displacement dt0 * max|vel| is `reach` cells."""
+    rho, u, v, *_ = W.random_state(R, seed=seed, vel_scale=1.0)
+    dt0 = np.sqrt(np.float32(R * R)) * np.float32(dt)
+    scale = np.float32(reach / dt0 / max(np.abs(u).max(), np.abs(v).max()))
+    return (rho, (u * scale).astype(np.float32), (v * scale).astype(np.float32)), (None, None, None)
+
+
+def fast_cell_start(R, seed, fast):
+    rho, u, v, *_ = W.random_state(R, seed=seed, vel_scale=0.002)
+    u, v = u.copy(), v.copy()
+    u[R // 3, R // 2], v[R // 3, R // 2] = fast
+    return (rho, u, v), (None, None, None)
+
+
+@pytest.mark.parametrize("mode", [fs2d_amd.LAUNCH_GRAPH, fs2d_amd.LAUNCH_EAGER])
+def test_batch_scatter_radii_and_fast_cells(mode, oracle_lib):
+    """The batched density scatter: simulations whose sources reach 0.7 to 3.7
+    cells (every radius of the window path) beside simulations with one fast
+    cell (u = 5 and 1e6: the batched sorted path), three steps, every
+    simulation bit for bit against the oracle."""
+    sims = [(66, 8, 0.1, 1e-4, 1e-5), (70, 8, 0.1, 0.0, 0.0), (130, 8, 0.05, 1e-4, 0.0), (98, 8, 0.1, 0.0, 1e-5),
+            (258, 8, 0.1, 1e-5, 1e-6), (34, 8, 0.1, 0.0, 0.0), (130, 16, 0.1, 1e-5, 1e-6)]
+    starts = [scaled_start(66, 0.1, 0.7, 1), scaled_start(70, 0.1, 1.5, 2), scaled_start(130, 0.05, 2.5, 3),
+              scaled_start(98, 0.1, 3.7, 4), fast_cell_start(258, 5, (5.0, 0.0)), fast_cell_start(34, 6, (1e6, -2.0)),
+              scaled_start(130, 0.1, 0.3, 7)]
+    run_and_check(sims, 0, mode, (1, 3), oracle_lib, starts=starts)
+
+
+def test_batch_launch_count_includes_the_scatter():
+    sims = [(66, 8, 0.1, 1e-4, 1e-5), (130, 16, 0.05, 1e-5, 1e-6)]
+    with fs2d_amd.Batch(sims, jacobi_block=4) as b:
+        for k, (R, *_rest) in enumerate(sims):
+            b.upload(k, *W.batch_state(R, seed=1, sim_index=k))
+            b.set_sources(k)
+        b.step_n(1)
+        info = b.info()
+    targets = sum((R - 1) ** 2 for R, *_ in sims)
+    passes = -(-targets.bit_length() // 9)
+    # add, 4 diffuse launches (16 sweeps / 4), the scatter (window, gate, keys, histograms, scans, scatters,
+    # bounds, merge),
+    # 2 x (divergence, 4 pressure launches, gradient, fold), gather
+    assert info["launches_per_step"] == 1 + 4 + (4 + 3 * passes) + 2 * (3 + 4) + 1, info
