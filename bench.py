@@ -158,6 +158,8 @@ def run_batch(count: int, steps: int, warmup: int):
             "unit": "steps/s (all simulations)", "ms_per_step": round(sec * 1e3, 3),
             "cell_updates_per_s": round(cells / sec, 1), "jacobi_depth": info["jacobi_depth"],
             "tile_rows": info["tile_rows"], "launches_per_step": info["launches_per_step"],
+            "persistent_blocks_per_cu": {"jacobi": info["jacobi_blocks_per_cu"], "stages": info["stage_blocks_per_cu"],
+                                         "note": "4 waves per block; 0 = one block per work entry (tuned)"},
             "graph_replayed": info["graphs"] > 0,
             "workload": "sides uniform over {64,96,128,192,256,384,512}, dt ~ U[0.02,0.1], K in {20,40}, "
                         "16-vortex starts, diffusion 1e-5, viscosity 1e-6, Jacobi order"}

@@ -70,10 +70,13 @@ constexpr int kSimSlotStride = 32;    // words between slots (separate cache lin
 
 __device__ __forceinline__ int xcd_block(int b, int nb) { return jac::xcd_block(b, nb); }
 
-// Work entry of a single-pass stage: x = simulation, y = (by << 16) | bx.
-__device__ __forceinline__ bool cell_of(const BatchSim* __restrict__ sims, const int2* __restrict__ work, int nb,
-                                        int& s, int& i, int& j, int& blk) {
-    const int2 w = work[xcd_block(blockIdx.x, nb)];
+// Single-pass stages are persistent: block b walks entries b, b + G, ... of
+// the work list (G blocks, a multiple of 8: each block's entries then map,
+// through xcd_block, into one contiguous run of the list per XCD).
+// Entry e: x = simulation, y = (by << 16) | bx.
+__device__ __forceinline__ bool cell_of(const BatchSim* __restrict__ sims, const int2* __restrict__ work, int e,
+                                        int nb, int& s, int& i, int& j, int& blk) {
+    const int2 w = work[xcd_block(e, nb)];
     s = w.x;
     const int bx = w.y & 0xffff, by = w.y >> 16;
     j = bx * CBX + threadIdx.x + 1;
@@ -84,37 +87,43 @@ __device__ __forceinline__ bool cell_of(const BatchSim* __restrict__ sims, const
 }
 
 __global__ __launch_bounds__(CBX* CBY) void kb_add_sources(const BatchSim* __restrict__ sims,
-                                                           const int2* __restrict__ work, Roles r) {
-    int s, i, j, blk;
-    if (!cell_of(sims, work, gridDim.x, s, i, j, blk)) return;
-    const BatchSim& S = sims[s];
-    const size_t o = S.off;
-    cells::add_sources(r.a[A_RHO] + o, r.a[A_X0] + o, r.a[A_RS] + o, S.P, S.dt, i, j);
-    cells::add_sources_split(r.a[A_UN] + o, r.a[A_X0U] + o, r.a[A_U] + o, r.a[A_US] + o, S.R, S.P, S.dt, i, j);
-    cells::add_sources_split(r.a[A_VN] + o, r.a[A_X0V] + o, r.a[A_V] + o, r.a[A_VS] + o, S.R, S.P, S.dt, i, j);
+                                                           const int2* __restrict__ work, int nwork, Roles r) {
+    for (int e = blockIdx.x; e < nwork; e += gridDim.x) {
+        int s, i, j, blk;
+        if (!cell_of(sims, work, e, nwork, s, i, j, blk)) continue;
+        const BatchSim& S = sims[s];
+        const size_t o = S.off;
+        cells::add_sources(r.a[A_RHO] + o, r.a[A_X0] + o, r.a[A_RS] + o, S.P, S.dt, i, j);
+        cells::add_sources_split(r.a[A_UN] + o, r.a[A_X0U] + o, r.a[A_U] + o, r.a[A_US] + o, S.R, S.P, S.dt, i, j);
+        cells::add_sources_split(r.a[A_VN] + o, r.a[A_X0V] + o, r.a[A_V] + o, r.a[A_VS] + o, S.R, S.P, S.dt, i, j);
+    }
 }
 
 // Block maximum of the largest finite |u|, |v| into one of the simulation's
-// slots (k_fold folds them).
+// slots (k_fold folds them).  Ends with a barrier, so a persistent block may
+// reduce its next entry at once.
 __device__ __forceinline__ void speed_to_slot(float m, unsigned* __restrict__ slots, int s, int blk) {
     m = cells::block_max(m);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const unsigned slot = (unsigned)((blk & 0xffff) + 131 * (blk >> 16)) % kSimSlots;
         atomicMax(slots + ((size_t)s * kSimSlots + slot) * kSimSlotStride, __float_as_uint(m));
     }
+    __syncthreads();
 }
 
 __global__ __launch_bounds__(CBX* CBY) void kb_max_speed(const BatchSim* __restrict__ sims,
-                                                         const int2* __restrict__ work, Roles r,
+                                                         const int2* __restrict__ work, int nwork, Roles r,
                                                          unsigned* __restrict__ slots) {
-    int s, i, j, blk;
-    const bool in = cell_of(sims, work, gridDim.x, s, i, j, blk);
-    float m = 0.f;
-    if (in) {
-        const size_t o = sims[s].off + (size_t)i * sims[s].P + j;
-        m = cells::fold_speed(0.f, r.a[A_U][o], r.a[A_V][o]);
+    for (int e = blockIdx.x; e < nwork; e += gridDim.x) {
+        int s, i, j, blk;
+        const bool in = cell_of(sims, work, e, nwork, s, i, j, blk);
+        float m = 0.f;
+        if (in) {
+            const size_t o = sims[s].off + (size_t)i * sims[s].P + j;
+            m = cells::fold_speed(0.f, r.a[A_U][o], r.a[A_V][o]);
+        }
+        speed_to_slot(m, slots, s, blk);
     }
-    speed_to_slot(m, slots, s, blk);
 }
 
 // One 64-lane block per simulation: fold its slots into max_bits[s] and clear them.
@@ -131,47 +140,63 @@ __global__ __launch_bounds__(64) void kb_fold_speed(unsigned* __restrict__ max_b
 }
 
 __global__ __launch_bounds__(CBX* CBY) void kb_gather_velocity(const BatchSim* __restrict__ sims,
-                                                               const int2* __restrict__ work, Roles r) {
-    int s, i, j, blk;
-    if (!cell_of(sims, work, gridDim.x, s, i, j, blk)) return;
-    const BatchSim& S = sims[s];
-    const size_t o = S.off;
-    cells::gather_velocity(r.a[A_T1] + o, r.a[A_T2] + o, r.a[A_U] + o, r.a[A_V] + o, S.R, S.P, S.dt0, i, j);
+                                                               const int2* __restrict__ work, int nwork, Roles r) {
+    for (int e = blockIdx.x; e < nwork; e += gridDim.x) {
+        int s, i, j, blk;
+        if (!cell_of(sims, work, e, nwork, s, i, j, blk)) continue;
+        const BatchSim& S = sims[s];
+        const size_t o = S.off;
+        cells::gather_velocity(r.a[A_T1] + o, r.a[A_T2] + o, r.a[A_U] + o, r.a[A_V] + o, S.R, S.P, S.dt0, i, j);
+    }
 }
 
 __global__ __launch_bounds__(CBX* CBY) void kb_divergence(const BatchSim* __restrict__ sims,
-                                                          const int2* __restrict__ work, Roles r) {
-    int s, i, j, blk;
-    if (!cell_of(sims, work, gridDim.x, s, i, j, blk)) return;
-    const BatchSim& S = sims[s];
-    const size_t o = S.off;
-    // the pressure iterate needs no zero fill: its first Jacobi launch reads zeros itself
-    cells::divergence(r.a[A_D] + o, r.a[A_U] + o, r.a[A_V] + o, S.R, S.P, S.neg_half_h, nullptr, i, j);
+                                                          const int2* __restrict__ work, int nwork, Roles r) {
+    for (int e = blockIdx.x; e < nwork; e += gridDim.x) {
+        int s, i, j, blk;
+        if (!cell_of(sims, work, e, nwork, s, i, j, blk)) continue;
+        const BatchSim& S = sims[s];
+        const size_t o = S.off;
+        // the pressure iterate needs no zero fill: its first Jacobi launch reads zeros itself
+        cells::divergence(r.a[A_D] + o, r.a[A_U] + o, r.a[A_V] + o, S.R, S.P, S.neg_half_h, nullptr, i, j);
+    }
 }
 
 __global__ __launch_bounds__(CBX* CBY) void kb_gradient(const BatchSim* __restrict__ sims,
-                                                        const int2* __restrict__ work, Roles r,
+                                                        const int2* __restrict__ work, int nwork, Roles r,
                                                         unsigned* __restrict__ slots) {
-    int s, i, j, blk;
-    const bool in = cell_of(sims, work, gridDim.x, s, i, j, blk);
-    float m = 0.f;
-    if (in) {
-        const BatchSim& S = sims[s];
-        const size_t o = S.off;
-        m = cells::gradient(r.a[A_U] + o, r.a[A_V] + o, r.a[A_P] + o, S.R, S.P, S.h, false, i, j);
+    for (int e = blockIdx.x; e < nwork; e += gridDim.x) {
+        int s, i, j, blk;
+        const bool in = cell_of(sims, work, e, nwork, s, i, j, blk);
+        float m = 0.f;
+        if (in) {
+            const BatchSim& S = sims[s];
+            const size_t o = S.off;
+            m = cells::gradient(r.a[A_U] + o, r.a[A_V] + o, r.a[A_P] + o, S.R, S.P, S.h, false, i, j);
+        }
+        speed_to_slot(m, slots, s, blk);
     }
-    speed_to_slot(m, slots, s, blk);
 }
 
 // Jacobi launch j of M fused sweeps over (simulation, field, tile) entries:
 // x = simulation | field << 24, y = tile (row-major over tiles_y x tiles_x).
+// Persistent: wave g of the grid walks entries g, g + waves, ... (waves = all
+// of the grid's), each entry one tile of one simulation's field; no
+// block-wide barrier, so a wave leaves when its entries are done.
+template <int M, int TY, int KIND>
+__device__ __forceinline__ void kb_jacobi_entry(const BatchSim* __restrict__ sims, const int2 w, Roles r, int j);
+
 template <int M, int TY, int KIND>
 __global__ __launch_bounds__(64 * jac::JW_WAVES) void kb_jacobi(const BatchSim* __restrict__ sims,
                                                                const int2* __restrict__ work, int n_work, Roles r,
                                                                int j) {
-    const int t = xcd_block(blockIdx.x, gridDim.x) * jac::JW_WAVES + threadIdx.x / 64;
-    if (t >= n_work) return;  // whole wave
-    const int2 w = work[t];
+    const int waves = (int)gridDim.x * jac::JW_WAVES;
+    for (int t = xcd_block(blockIdx.x, gridDim.x) * jac::JW_WAVES + threadIdx.x / 64; t < n_work; t += waves)
+        kb_jacobi_entry<M, TY, KIND>(sims, work[t], r, j);
+}
+
+template <int M, int TY, int KIND>
+__device__ __forceinline__ void kb_jacobi_entry(const BatchSim* __restrict__ sims, const int2 w, Roles r, int j) {
     const int s = w.x & 0xffffff, f = w.x >> 24;
     const BatchSim& S = sims[s];
     constexpr int OX = 64 - 2 * M;
@@ -223,9 +248,19 @@ __global__ __launch_bounds__(256) void kb_fill(uint32_t* __restrict__ p, size_t 
 constexpr int kDepths[] = {1, 2, 4, 8};
 constexpr int kTileRows[] = {16, 32};
 
+// Blocks of a persistent launch over `n` entries of `per_block` each, at most
+// `limit` (a multiple of 8 when it binds, so that every block's entries fall
+// in one XCD's run of the list).
+unsigned persistent_grid(int n, int per_block, int limit) {
+    const int need = (n + per_block - 1) / per_block;
+    if (limit <= 0 || need <= limit) return static_cast<unsigned>(std::max(need, 1));
+    return static_cast<unsigned>(std::max(8, limit - limit % 8));
+}
+
 template <int M, int TY>
-void launch_kb_jacobi_mt(hipStream_t st, int kind, const BatchSim* sims, const int2* w, int n, Roles r, int j) {
-    const dim3 grid((n + jac::JW_WAVES - 1) / jac::JW_WAVES);
+void launch_kb_jacobi_mt(hipStream_t st, int kind, const BatchSim* sims, const int2* w, int n, Roles r, int j,
+                         int limit) {
+    const dim3 grid(persistent_grid(n, jac::JW_WAVES, limit));
     if (kind == SWEEP_DIFFUSE)
         hipLaunchKernelGGL((kb_jacobi<M, TY, SWEEP_DIFFUSE>), grid, dim3(64 * jac::JW_WAVES), 0, st, sims, w, n, r, j);
     else
@@ -234,21 +269,22 @@ void launch_kb_jacobi_mt(hipStream_t st, int kind, const BatchSim* sims, const i
 }
 
 template <int TY>
-bool launch_kb_jacobi_t(hipStream_t st, int M, int kind, const BatchSim* sims, const int2* w, int n, Roles r, int j) {
+bool launch_kb_jacobi_t(hipStream_t st, int M, int kind, const BatchSim* sims, const int2* w, int n, Roles r, int j,
+                        int limit) {
     switch (M) {
-        case 1: launch_kb_jacobi_mt<1, TY>(st, kind, sims, w, n, r, j); return true;
-        case 2: launch_kb_jacobi_mt<2, TY>(st, kind, sims, w, n, r, j); return true;
-        case 4: launch_kb_jacobi_mt<4, TY>(st, kind, sims, w, n, r, j); return true;
-        case 8: launch_kb_jacobi_mt<8, TY>(st, kind, sims, w, n, r, j); return true;
+        case 1: launch_kb_jacobi_mt<1, TY>(st, kind, sims, w, n, r, j, limit); return true;
+        case 2: launch_kb_jacobi_mt<2, TY>(st, kind, sims, w, n, r, j, limit); return true;
+        case 4: launch_kb_jacobi_mt<4, TY>(st, kind, sims, w, n, r, j, limit); return true;
+        case 8: launch_kb_jacobi_mt<8, TY>(st, kind, sims, w, n, r, j, limit); return true;
         default: return false;
     }
 }
 
 hipError_t launch_kb_jacobi(hipStream_t st, int M, int TY, int kind, const BatchSim* sims, const int2* w, int n,
-                            Roles r, int j) {
+                            Roles r, int j, int limit) {
     if (n <= 0) return hipSuccess;
-    const bool ok = TY == 16 ? launch_kb_jacobi_t<16>(st, M, kind, sims, w, n, r, j)
-                             : (TY == 32 ? launch_kb_jacobi_t<32>(st, M, kind, sims, w, n, r, j) : false);
+    const bool ok = TY == 16 ? launch_kb_jacobi_t<16>(st, M, kind, sims, w, n, r, j, limit)
+                             : (TY == 32 ? launch_kb_jacobi_t<32>(st, M, kind, sims, w, n, r, j, limit) : false);
     return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
@@ -293,6 +329,10 @@ struct fs2d_batch {
     void* scatter_scratch = nullptr;     // its sorted path's buffers
     int* scatter_flag = nullptr;         // its sorted-path gate
     BatchScatter scatter{};
+    int cus = 256;                // compute units of the device
+    int jac_per_cu = 0;           // persistent Jacobi blocks per CU (0 until tuned: one block per 4 entries)
+    int cell_per_cu = 0;          // persistent single-pass blocks per CU (0 until tuned: one per entry)
+    bool occupancy_fixed = false; // set by fs2d_hip_batch_set_occupancy: the tuner keeps them
     JacobiConfig cfg;
     bool tuned = false;
     bool speed_valid = false;
@@ -317,6 +357,9 @@ int bfail(fs2d_batch* b, int code, const std::string& msg) {
     } while (0)
 
 float cells_of(int R) { return static_cast<float>(static_cast<size_t>(R) * static_cast<size_t>(R)); }
+
+int jac_limit(const fs2d_batch* b) { return b->jac_per_cu > 0 ? b->cus * b->jac_per_cu : 0; }
+dim3 cell_grid(const fs2d_batch* b) { return dim3(persistent_grid(b->cell_blocks, 1, b->cell_per_cu > 0 ? b->cus * b->cell_per_cu : 0)); }
 
 void free_config(JacobiConfig& c) {
     if (c.diffuse_work) (void)hipFree(c.diffuse_work);
@@ -380,19 +423,20 @@ int solve(fs2d_batch* b, int kind) {
     const auto& counts = kind == SWEEP_DIFFUSE ? c.diffuse_count : c.pressure_count;
     const int2* w = kind == SWEEP_DIFFUSE ? c.diffuse_work : c.pressure_work;
     for (int j = 0; j < static_cast<int>(counts.size()); ++j)
-        FS2D_BHIP(b, launch_kb_jacobi(b->stream, c.M, c.TY, kind, b->sims, w, counts[j], b->roles, j));
+        FS2D_BHIP(b, launch_kb_jacobi(b->stream, c.M, c.TY, kind, b->sims, w, counts[j], b->roles, j, jac_limit(b)));
     return FS2D_OK;
 }
 
 hipError_t cell_launch_check() { return hipGetLastError(); }
 
 int project(fs2d_batch* b) {
-    const dim3 grid(b->cell_blocks), blk(CBX, CBY);
-    hipLaunchKernelGGL(kb_divergence, grid, blk, 0, b->stream, b->sims, b->cell_work, b->roles);
+    const dim3 grid = cell_grid(b), blk(CBX, CBY);
+    hipLaunchKernelGGL(kb_divergence, grid, blk, 0, b->stream, b->sims, b->cell_work, b->cell_blocks, b->roles);
     FS2D_BHIP(b, cell_launch_check());
     int rc = solve(b, SWEEP_PRESSURE);
     if (rc) return rc;
-    hipLaunchKernelGGL(kb_gradient, grid, blk, 0, b->stream, b->sims, b->cell_work, b->roles, b->slots);
+    hipLaunchKernelGGL(kb_gradient, grid, blk, 0, b->stream, b->sims, b->cell_work, b->cell_blocks, b->roles,
+                       b->slots);
     hipLaunchKernelGGL(kb_fold_speed, dim3(b->count), dim3(64), 0, b->stream, b->max_bits, b->slots);
     FS2D_BHIP(b, cell_launch_check());
     return FS2D_OK;
@@ -412,7 +456,7 @@ int time_config(fs2d_batch* b, float* est_ms) {
         float best = 0.f;
         for (int rep = 0; rep < 3; ++rep) {
             FS2D_BHIP(b, hipEventRecord(b->ev0, b->stream));
-            FS2D_BHIP(b, launch_kb_jacobi(b->stream, c.M, c.TY, kind, b->sims, w, n0, b->roles, 0));
+            FS2D_BHIP(b, launch_kb_jacobi(b->stream, c.M, c.TY, kind, b->sims, w, n0, b->roles, 0, jac_limit(b)));
             FS2D_BHIP(b, hipEventRecord(b->ev1, b->stream));
             FS2D_BHIP(b, hipEventSynchronize(b->ev1));
             float ms = 0.f;
@@ -458,23 +502,67 @@ int tune(fs2d_batch* b) {
     int rc = build_config(b, chosen.M, chosen.TY, &b->cfg);
     if (rc) return rc;
     if ((rc = upload_sims(b, chosen.M))) return rc;
+    if (b->occupancy_fixed) {
+        b->tuned = true;
+        return FS2D_OK;
+    }
+    // waves per CU of the persistent launches: Jacobi blocks (4 waves) per CU
+    // timed on the same launches as above, single-pass blocks (4 waves) per
+    // CU on the divergence (it only writes the divergence arena, which every
+    // projection recomputes first); 0 is one block per work entry
+    float best_j = 0.f;
+    int pick_j = 0;
+    for (int f : {0, 1, 2, 4, 8}) {
+        b->jac_per_cu = f;
+        float est = 0.f;
+        if ((rc = time_config(b, &est))) return rc;
+        if (f == 0 || est < best_j) {
+            best_j = est;
+            pick_j = f;
+        }
+    }
+    b->jac_per_cu = pick_j;
+    float best_c = 0.f;
+    int pick_c = 0;
+    for (int f : {0, 2, 4, 8}) {
+        b->cell_per_cu = f;
+        float t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            FS2D_BHIP(b, hipEventRecord(b->ev0, b->stream));
+            hipLaunchKernelGGL(kb_divergence, cell_grid(b), dim3(CBX, CBY), 0, b->stream, b->sims, b->cell_work,
+                               b->cell_blocks, b->roles);
+            FS2D_BHIP(b, hipGetLastError());
+            FS2D_BHIP(b, hipEventRecord(b->ev1, b->stream));
+            FS2D_BHIP(b, hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            FS2D_BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (rep == 1 || (rep > 1 && ms < t)) t = ms;
+        }
+        if (f == 0 || t < best_c) {
+            best_c = t;
+            pick_c = f;
+        }
+    }
+    b->cell_per_cu = pick_c;
     b->tuned = true;
     return FS2D_OK;
 }
 
 // One step of every simulation (fluid_solver_cpu::solve, Jacobi order).
 int step(fs2d_batch* b) {
-    const dim3 grid(b->cell_blocks), blk(CBX, CBY);
+    const dim3 blk(CBX, CBY);
     int rc;
     if (!b->speed_valid) {
-        hipLaunchKernelGGL(kb_max_speed, grid, blk, 0, b->stream, b->sims, b->cell_work, b->roles, b->slots);
+        hipLaunchKernelGGL(kb_max_speed, cell_grid(b), blk, 0, b->stream, b->sims, b->cell_work, b->cell_blocks,
+                           b->roles, b->slots);
         hipLaunchKernelGGL(kb_fold_speed, dim3(b->count), dim3(64), 0, b->stream, b->max_bits, b->slots);
         FS2D_BHIP(b, cell_launch_check());
         b->speed_valid = true;
     }
     // sources of all three fields; u and v go to fresh iterates so that the
     // density scatter still reads the pre-step velocity (:16-24)
-    hipLaunchKernelGGL(kb_add_sources, grid, blk, 0, b->stream, b->sims, b->cell_work, b->roles);
+    hipLaunchKernelGGL(kb_add_sources, cell_grid(b), blk, 0, b->stream, b->sims, b->cell_work, b->cell_blocks,
+                       b->roles);
     FS2D_BHIP(b, cell_launch_check());
     if (!b->tuned && (rc = tune(b))) return rc;
     if ((rc = solve(b, SWEEP_DIFFUSE))) return rc;
@@ -484,7 +572,8 @@ int step(fs2d_batch* b) {
     std::swap(b->roles.a[A_U], b->roles.a[A_UN]);
     std::swap(b->roles.a[A_V], b->roles.a[A_VN]);
     if ((rc = project(b))) return rc;
-    hipLaunchKernelGGL(kb_gather_velocity, grid, blk, 0, b->stream, b->sims, b->cell_work, b->roles);
+    hipLaunchKernelGGL(kb_gather_velocity, cell_grid(b), blk, 0, b->stream, b->sims, b->cell_work, b->cell_blocks,
+                       b->roles);
     FS2D_BHIP(b, cell_launch_check());
     std::swap(b->roles.a[A_U], b->roles.a[A_T1]);
     std::swap(b->roles.a[A_V], b->roles.a[A_T2]);
@@ -691,6 +780,12 @@ static int fs2d_hip_batch_create_impl(int count, const fs2d_batch_sim* sims, con
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(e, "hipStreamCreate");
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            b->cus = cus;
+    }
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
     const size_t bytes = sizeof(float) * off;
@@ -830,6 +925,27 @@ static int fs2d_hip_batch_info_impl(fs2d_batch* b, int* depth, int* tile_rows, i
     }
     if (graphs) *graphs = static_cast<int>(b->graphs.size());
     if (interior_cells) *interior_cells = b->interior;
+    return FS2D_OK;
+}
+
+int fs2d_hip_batch_set_occupancy(fs2d_batch* b, int jacobi_blocks_per_cu, int stage_blocks_per_cu) {
+    if (!b || jacobi_blocks_per_cu < 0 || jacobi_blocks_per_cu > 64 || stage_blocks_per_cu < 0 ||
+        stage_blocks_per_cu > 64)
+        return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_set_occupancy: blocks per CU must be in [0, 64]");
+    if (b->steps_done > 0)
+        return bfail(b, FS2D_ERR_STATE, "fs2d_hip_batch_set_occupancy: only before the first step");
+    b->jac_per_cu = jacobi_blocks_per_cu;
+    b->cell_per_cu = stage_blocks_per_cu;
+    b->occupancy_fixed = true;
+    return FS2D_OK;
+}
+
+int fs2d_hip_batch_occupancy(fs2d_batch* b, int* jacobi_blocks_per_cu, int* stage_blocks_per_cu,
+                             int* compute_units) {
+    if (!b) return bfail(b, FS2D_ERR_ARGUMENT, "fs2d_hip_batch_occupancy: NULL batch");
+    if (jacobi_blocks_per_cu) *jacobi_blocks_per_cu = b->jac_per_cu;
+    if (stage_blocks_per_cu) *stage_blocks_per_cu = b->cell_per_cu;
+    if (compute_units) *compute_units = b->cus;
     return FS2D_OK;
 }
 

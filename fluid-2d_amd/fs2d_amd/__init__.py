@@ -84,6 +84,8 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_batch_info": (_i, [H, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
                                      ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_longlong)]),
         "fs2d_hip_batch_last_error": (ctypes.c_char_p, [H]),
+        "fs2d_hip_batch_occupancy": (_i, [H, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+        "fs2d_hip_batch_set_occupancy": (_i, [H, _i, _i]),
         "fs2d_hip_upload_device": (_i, [H, _p, _p, _p]),
         "fs2d_hip_download_device": (_i, [H, _p, _p, _p]),
         "fs2d_hip_stage_add_sources": (_i, [H, _i, _f]),
@@ -284,7 +286,7 @@ EXPORTED_SYMBOLS = (
     "fs2d_hip_set_sources", "fs2d_hip_step_n", "fs2d_hip_download", "fs2d_hip_sync",
     "fs2d_hip_graph_count", "fs2d_hip_copy_bandwidth", "fs2d_hip_batch_create", "fs2d_hip_batch_destroy", "fs2d_hip_batch_upload",
     "fs2d_hip_batch_set_sources", "fs2d_hip_batch_step_n", "fs2d_hip_batch_download", "fs2d_hip_batch_sync",
-    "fs2d_hip_batch_info", "fs2d_hip_batch_last_error",
+    "fs2d_hip_batch_info", "fs2d_hip_batch_last_error", "fs2d_hip_batch_occupancy", "fs2d_hip_batch_set_occupancy",
     "fs2d_net_create", "fs2d_net_destroy", "fs2d_net_param_count", "fs2d_net_get_params", "fs2d_net_set_params",
     "fs2d_net_get_grads", "fs2d_net_train_step", "fs2d_net_predict_host", "fs2d_net_stream", "fs2d_net_last_error",
     "fs2d_net_time_forward", "fs2d_net_forward_flops",
@@ -385,13 +387,20 @@ class Batch:
         self._ck(self._L.fs2d_hip_batch_download(self._h, sim, *(o.ctypes.data for o in out)))
         return out
 
+    def set_occupancy(self, jacobi_blocks_per_cu: int, stage_blocks_per_cu: int) -> None:
+        """Fix the persistent launches' blocks per CU (before the first step)."""
+        self._ck(self._L.fs2d_hip_batch_set_occupancy(self._h, jacobi_blocks_per_cu, stage_blocks_per_cu))
+
     def info(self) -> dict:
         d, t, l, g = _i(), _i(), _i(), _i()
         c = ctypes.c_longlong()
         self._ck(self._L.fs2d_hip_batch_info(self._h, ctypes.byref(d), ctypes.byref(t), ctypes.byref(l),
                                             ctypes.byref(g), ctypes.byref(c)))
+        jb, sb, cu = _i(), _i(), _i()
+        self._ck(self._L.fs2d_hip_batch_occupancy(self._h, ctypes.byref(jb), ctypes.byref(sb), ctypes.byref(cu)))
         return {"jacobi_depth": d.value, "tile_rows": t.value, "launches_per_step": l.value, "graphs": g.value,
-                "interior_cells": c.value}
+                "interior_cells": c.value, "jacobi_blocks_per_cu": jb.value, "stage_blocks_per_cu": sb.value,
+                "compute_units": cu.value}
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:

@@ -232,6 +232,14 @@ int fs2d_hip_batch_sync(fs2d_batch* b);
  * kernel launches per step, captured step graphs, total interior cells. */
 int fs2d_hip_batch_info(fs2d_batch* b, int* jacobi_depth, int* tile_rows, int* launches_per_step, int* graphs,
                         long long* interior_cells);
+/* The persistent launches' grid, as the autotuner chose it on the first step:
+ * blocks (of 4 waves) per compute unit for the Jacobi launches and for the
+ * single-pass stages (0: one block per work entry), and the compute units. */
+int fs2d_hip_batch_occupancy(fs2d_batch* b, int* jacobi_blocks_per_cu, int* stage_blocks_per_cu,
+                             int* compute_units);
+/* Fix those two figures instead of tuning them (before the first step only;
+ * 0..64, 0 = one block per work entry).  Results do not depend on them. */
+int fs2d_hip_batch_set_occupancy(fs2d_batch* b, int jacobi_blocks_per_cu, int stage_blocks_per_cu);
 const char* fs2d_hip_batch_last_error(const fs2d_batch* b);
 
 #ifdef __cplusplus

@@ -29,11 +29,13 @@ def start(k, R):
     return st, src
 
 
-def run_and_check(sims, block, mode, steps, oracle_lib, starts=None):
+def run_and_check(sims, block, mode, steps, oracle_lib, starts=None, occupancy=None):
     starts = starts or [start(k, s[0]) for k, s in enumerate(sims)]
     want = [[x.copy() for x in st] for st, _ in starts]
     z = [np.zeros((s[0], s[0]), np.float32) for s in sims]
     with fs2d_amd.Batch(sims, jacobi_block=block, launch_mode=mode) as b:
+        if occupancy is not None:
+            b.set_occupancy(*occupancy)
         for k, (st, src) in enumerate(starts):
             b.upload(k, *st)
             b.set_sources(k, *src)
@@ -64,6 +66,8 @@ def test_batch_matches_oracle(block, mode, oracle_lib):
     else:
         assert info["jacobi_depth"] in (1, 2, 4)  # 8 does not divide 8 at least twice
     assert info["tile_rows"] in (16, 32)
+    assert info["jacobi_blocks_per_cu"] in (0, 1, 2, 4, 8) and info["stage_blocks_per_cu"] in (0, 2, 4, 8)
+    assert info["compute_units"] > 0
     assert info["interior_cells"] == sum((R - 2) ** 2 for R, *_ in SIMS)
     if mode == fs2d_amd.LAUNCH_GRAPH:
         assert info["graphs"] >= 1
@@ -142,3 +146,12 @@ def test_batch_launch_count_includes_the_scatter():
     # bounds, merge),
     # 2 x (divergence, 4 pressure launches, gradient, fold), gather
     assert info["launches_per_step"] == 1 + 4 + (4 + 3 * passes) + 2 * (3 + 4) + 1, info
+
+
+@pytest.mark.parametrize("occupancy", [(1, 1), (2, 8), (0, 0)])
+def test_batch_persistent_grids(occupancy, oracle_lib):
+    """Fixed persistent grids: one block per CU (every wave and block walks
+    many entries), a middle setting, and one block per entry -- the same
+    bits as the oracle, and the setting is kept."""
+    info = run_and_check(SIMS, 2, fs2d_amd.LAUNCH_GRAPH, (1, 3), oracle_lib, occupancy=occupancy)
+    assert (info["jacobi_blocks_per_cu"], info["stage_blocks_per_cu"]) == occupancy

@@ -66,14 +66,19 @@ def test_whole_steps_against_reference_digests(name):
             check_fields(s.download(), rec["steps"][str(step)], f"{name} after {step} steps")
 
 
-def test_batched_workload_against_oracle_digests():
+@pytest.mark.parametrize("occupancy", [None, (1, 1)])
+def test_batched_workload_against_oracle_digests(occupancy):
     """The full batched workload (256 simulations of sides 64..512, uneven dt and
-    sweep counts), two steps in the Jacobi order, every simulation."""
+    sweep counts), two steps in the Jacobi order, every simulation: with the
+    tuned persistent grids, and with one block per CU (every block and wave
+    walks hundreds of work entries)."""
     rec = DIGESTS["config4_batch256"]
     cfg = W.batch_config(rec["count"], seed=rec["seed"])
     assert [(R, K, dt) for R, K, dt in cfg] == [(x["R"], x["K"], x["dt"]) for x in rec["sims"]]
     sims = [(R, K, dt, rec["diffusion"], rec["viscosity"]) for R, K, dt in cfg]
     with fs2d_amd.Batch(sims) as b:
+        if occupancy:
+            b.set_occupancy(*occupancy)
         for k, (R, K, dt) in enumerate(cfg):
             st = W.batch_state(R, seed=rec["seed"], sim_index=k)
             check_inputs(st, rec["sims"][k], f"simulation {k}")
