@@ -11,6 +11,7 @@
 #include "../../include/fs2d_render.h"
 #include "fs2d_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,6 +54,7 @@ struct fs2d_handle {
     // tile kernel, anything else the unit kernels.
     int gs_tw = 0, gs_block = 0, gs_lag = 0;
     std::map<std::pair<int, int>, GsPolicy> gs_policy;  // (kind, K) -> policy in use
+    std::map<std::tuple<int, int, int, int, int>, int> pipe_offsets;  // (Kd, Kp, tw, M, lag) -> pipeline offset
     int jacobi_kernel = FS2D_JACOBI_WAVE;
     std::map<std::pair<int, int>, int> jacobi_depth;  // (kind, K) -> sweeps per launch, when tuned
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
@@ -392,6 +394,42 @@ int advect_velocity(fs2d_handle* h, float dt) {
     return FS2D_OK;
 }
 
+// Levels the first projection's pressure solve trails the diffuses by, when
+// pipelined (gs_pipeline_offset), per (Kd, Kp, policy).
+int pipeline_offset_for(fs2d_handle* h, int Kd, const GsPlan& pp, const GsPolicy& c) {
+    const auto key = std::make_tuple(Kd, pp.K, c.tw, c.M, c.lag);
+    auto it = h->pipe_offsets.find(key);
+    if (it != h->pipe_offsets.end()) return it->second;
+    const int off = gs_pipeline_offset(pp, h->R, Kd, c.tw, c.M);
+    h->pipe_offsets.emplace(key, off);
+    return off;
+}
+
+// A projection whose pressure solve (Gauss-Seidel units) computes the
+// divergence of (u, v) where each unit loads its right-hand side, so no
+// divergence pass runs and none is stored; otherwise project().
+int project_onload(fs2d_handle* h, int K) {
+    if (K > 0 && h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h)) {
+        const GsPlan* plan = nullptr;
+        FS2D_HIP(h, gs_plan_for(h, K, SWEEP_PRESSURE, &plan));
+        if (plan->M > 0) {
+            const Scalars s = scalars(h->R, 1.0f);
+            FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
+            SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 0);
+            A.du = h->u();
+            A.dv = h->v();
+            A.neg_half_h = s.neg_half_h;
+            FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
+            h->pressure_zero = false;
+            FS2D_HIP(h, launch_gradient(h->stream, h->u(), h->v(), h->f[F_P], h->R, h->P, s.h, 0, h->max_bits,
+                                        h->speed_slots));
+            h->speed_valid = true;
+            return FS2D_OK;
+        }
+    }
+    return project(h, K);
+}
+
 // The three diffuses of a step relaxed together (Gauss-Seidel unit plans):
 // density in place, u and v on fresh iterates so that the density scatter can
 // still read the pre-step velocity (src/fluid_solver_cpu.cpp:16-24).  Same
@@ -413,6 +451,31 @@ int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
         set.a[2] = sweep_args(h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_V, Kd, 0);
         const GsPlan* plan = nullptr;
         FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
+        if (Kp > 0 && plan->M > 0) {
+            // The first projection pipelined behind the diffuses: its pressure
+            // solve (the diffuse's launch policy, right-hand side the divergence
+            // of the diffused velocity, computed where each unit loads it) runs in
+            // the same level launches, `offset` levels behind.
+            const GsPolicy pol{plan->TW, plan->M, plan->lag};
+            const GsPlan* pplan = nullptr;
+            FS2D_HIP(h, gs_plan_of(h, Kp, pol, &pplan));
+            const int offset = pipeline_offset_for(h, Kd, *pplan, pol);
+            FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
+            SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], R, P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, Kp, 0);
+            A.du = h->f[F_UN];
+            A.dv = h->f[F_VN];
+            A.neg_half_h = scalars(R, 1.0f).neg_half_h;
+            FS2D_HIP(h, launch_gs_pipelined(h->stream, *plan, set, *pplan, A, offset));
+            if ((rc = advect_density(h, dt))) return rc;  // reads the pre-step u, v
+            std::swap(h->u(), h->f[F_UN]);
+            std::swap(h->v(), h->f[F_VN]);
+            h->pressure_zero = false;
+            FS2D_HIP(h, launch_gradient(h->stream, h->u(), h->v(), h->f[F_P], R, P, scalars(R, 1.0f).h, 0,
+                                        h->max_bits, h->speed_slots));
+            h->speed_valid = true;
+            if ((rc = advect_velocity(h, dt))) return rc;
+            return project_onload(h, Kp);
+        }
         FS2D_HIP(h, launch_gs_solve_set(h->stream, *plan, set));
     }
     if ((rc = advect_density(h, dt))) return rc;  // reads the pre-step u, v
@@ -990,6 +1053,23 @@ int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per
     return guarded(h, "fs2d_hip_gs_policy", [&] { return fs2d_hip_gs_policy_impl(h, kind, tile_cols, sweeps_per_unit, lag, levels); });
 }
 
+int fs2d_gs_pipeline_offset(int rows, int diffuse_iterations, int pressure_iterations, int tile_cols,
+                            int sweeps_per_unit, int* offset) {
+    if (rows < 3 || diffuse_iterations < 1 || pressure_iterations < 1 || !offset ||
+        (tile_cols != 64 && tile_cols != 128) ||
+        (sweeps_per_unit != 2 && sweeps_per_unit != 4 && sweeps_per_unit != 8))
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_gs_pipeline_offset: bad argument");
+    GsPlan pp{};
+    const hipError_t e = gs_unit_plan_build(&pp, rows, pressure_iterations, tile_cols, sweeps_per_unit, 4, false);
+    if (e != hipSuccess) {
+        gs_plan_destroy(&pp);
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_gs_pipeline_offset: plan too large");
+    }
+    *offset = gs_pipeline_offset(pp, rows, diffuse_iterations, tile_cols, sweeps_per_unit);
+    gs_plan_destroy(&pp);
+    return FS2D_OK;
+}
+
 static int fs2d_hip_copy_bandwidth_impl(long long bytes, int reps, float* gbs) {
     if (bytes < 1024 || reps < 1 || !gbs) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_copy_bandwidth: bad argument");
     const size_t n4 = static_cast<size_t>(bytes) / 16;
@@ -1035,8 +1115,23 @@ static int fs2d_hip_launches_per_step_impl(fs2d_handle* h, int* launches) {
     };
     const int d = solve_launches(Kd, SWEEP_DIFFUSE), p = solve_launches(Kp, SWEEP_PRESSURE);
     if (d < 0 || p < 0) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
-    // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather
     const bool shared = h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h);
+    const GsPlan* dplan = nullptr;
+    if (shared && Kd > 0 && Kp > 0 && gs_plan_for(h, Kd, SWEEP_DIFFUSE, &dplan) == hipSuccess && dplan->M > 0) {
+        // pipelined: add x3, fill p, the joint level launches, scatter, gradient +
+        // fold, gather, then fill p, the second solve's levels, gradient + fold
+        const GsPolicy pol{dplan->TW, dplan->M, dplan->lag};
+        const GsPlan* pplan = nullptr;
+        if (gs_plan_of(h, Kp, pol, &pplan) != hipSuccess) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
+        const int off = pipeline_offset_for(h, Kd, *pplan, pol);
+        std::vector<int> lv(dplan->level_of, dplan->level_of + dplan->levels);
+        for (int l = 0; l < pplan->levels; ++l) lv.push_back(pplan->level_of[l] + off);
+        std::sort(lv.begin(), lv.end());
+        const int joint = static_cast<int>(std::unique(lv.begin(), lv.end()) - lv.begin());
+        *launches = 3 + 1 + joint + scatter_launches(h->R) + 2 + 1 + 1 + p + 2;
+        return FS2D_OK;
+    }
+    // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather
     *launches = 3 + (shared ? d : 3 * d) + scatter_launches(h->R) + 2 * (2 + p) + 1;
     return FS2D_OK;
 }

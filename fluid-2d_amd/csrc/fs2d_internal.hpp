@@ -48,6 +48,12 @@ struct SweepArgs {
     int K;           // total sweeps of this solve
     int zero_start;  // iterate starts at 0 everywhere, ring included (pressure, :186)
     float div_lo;    // smallest nonzero |dividend| the FMA-corrected division takes
+    // Pressure with du, dv set: the right-hand side is the divergence of
+    // (du, dv), neg_half_h * (((uE - uW) + vS) - vN) (src/fluid_solver_cpu.cpp:188-193),
+    // computed where the solve loads it instead of read from b.
+    const float* du;
+    const float* dv;
+    float neg_half_h;
 };
 
 // Lower end of the dividends the FMA-corrected division handles (zero aside):
@@ -61,7 +67,7 @@ inline float division_floor(float d) {
 
 inline SweepArgs sweep_args(float* x, const float* b, int R, int P, int kind, float a, float denom, int mode, int K,
                             int zero_start) {
-    SweepArgs A{x, b, R, P, kind, a, denom, 0.f, ring_of(mode), K, zero_start, __builtin_inff()};
+    SweepArgs A{x, b, R, P, kind, a, denom, 0.f, ring_of(mode), K, zero_start, __builtin_inff(), nullptr, nullptr, 0.f};
     if (kind == SWEEP_DIFFUSE) {
         A.inv_denom = 1.0f / denom;
         A.div_lo = division_floor(denom);
@@ -72,7 +78,7 @@ inline SweepArgs sweep_args(float* x, const float* b, int R, int P, int kind, fl
 // Up to three independent solves with the same sweep count, relaxed by the
 // same level launches (e.g. the density, u and v diffuses of one step).
 struct SweepSet {
-    static constexpr int kMax = 3;
+    static constexpr int kMax = 4;
     SweepArgs a[kMax];
     int count;
     int units_per_field;  // set per launch
@@ -169,9 +175,13 @@ struct GsPlan {
     int levels;
     int2* work;            // device: (I << 16 | J, k) per tile-sweep, grouped by level
     int* host_offsets;     // levels + 1 offsets into work
+    int* level_of;         // host: I' + J' + kappa of each (non-empty) level
+    int2* host_work;       // host copy of work
 };
 hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW);
 hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int lag);
+// The same plan on the host only (no device work list) when upload is false.
+hipError_t gs_unit_plan_build(GsPlan* plan, int R, int K, int TW, int M, int lag, bool upload);
 void gs_plan_destroy(GsPlan* plan);
 hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a);
 // The same solve with the lagged unit kernel (M sweeps per unit, lag steps)
@@ -181,6 +191,17 @@ constexpr int kImplicitMaxSweeps = 4096;
 hipError_t launch_gs_solve_implicit(hipStream_t s, int R, int K, int TW, int M, int lag, const SweepArgs& a);
 // All solves of the set in shared level launches (unit plans; a tile plan takes one).
 hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set);
+// Two solves pipelined: the diffuse fields of `diffuse` (plan pd) and the
+// pressure solve p (plan pp, its right-hand side the divergence of the
+// diffused velocity, p.du / p.dv) in the same level launches, pressure level
+// l running with diffuse level l + offset.  offset = gs_pipeline_offset(...):
+// every pressure unit then runs after the diffuse units that write the
+// velocity its divergence reads.  Both plans are unit plans of one policy.
+hipError_t launch_gs_pipelined(hipStream_t s, const GsPlan& pd, const SweepSet& diffuse, const GsPlan& pp,
+                               const SweepArgs& p, int offset);
+// The smallest such offset for a pressure plan behind a diffuse solve of Kd
+// sweeps with tile columns TWd and Md sweeps per unit.
+int gs_pipeline_offset(const GsPlan& pp, int R, int Kd, int TWd, int Md);
 
 // Jacobi sweeps, `block` sweeps fused per launch, ping-ponging x -> scratch ->
 // x -> ...; after an odd number of launches the result is in scratch.

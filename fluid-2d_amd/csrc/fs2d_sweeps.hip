@@ -250,25 +250,60 @@ __device__ __forceinline__ int2 unit_of(const SweepSet& S, const int2* __restric
     return make_int2(0, 0);  // not reached: a level launches exactly its unit count
 }
 
-template <int TW, int M, int L, int KIND>
-__global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __restrict__ work) {
-    // blocks [f * units, (f + 1) * units) relax field f of the set, unit by unit
-    const int f = blockIdx.x / S.units_per_field;
-    const SweepArgs& A = S.a[f];
-    constexpr int NT = 64 * M;
-    constexpr int XH = GS_ROWS + 1 + M, XW = TW + 1 + M;            // iterate region
-    constexpr int XP = XW + (XW & 1);                                // even pitch: skew is conflict-free
-    constexpr int BH = GS_ROWS - 1 + M, BW = TW - 1 + M;             // right-hand side region
-    constexpr int BP = BW + (BW & 1);
-    constexpr int NX = (XH * XW + NT - 1) / NT, NBX = (BH * BW + NT - 1) / NT;
+// LDS of a unit: the iterate region X and the right-hand side region Bs.
+template <int TW, int M>
+struct UnitLds {
+    static constexpr int XH = GS_ROWS + 1 + M, XW = TW + 1 + M;  // iterate region
+    static constexpr int XP = XW + (XW & 1);                      // even pitch: skew is conflict-free
+    static constexpr int BH = GS_ROWS - 1 + M, BW = TW - 1 + M;   // right-hand side region
+    static constexpr int BP = BW + (BW & 1);
     // Guard space after each array absorbs the operand reads of inactive lanes
     // (their skewed columns run up to GS_ROWS + TW + M past the cell); lanes
     // that relax nothing in a step store nothing.
-    constexpr int GUARD = GS_ROWS + TW + 2 * M + 8;
-    __shared__ float X[XH * XP + GUARD];
-    __shared__ float Bs[BH * BP + GUARD];
+    static constexpr int GUARD = GS_ROWS + TW + 2 * M + 8;
+    static constexpr int NXS = XH * XP + GUARD, NBS = BH * BP + GUARD;
+};
 
-    const int2 wk = unit_of(S, work, blockIdx.x - f * S.units_per_field);
+template <int TW, int M, int L, int KIND>
+__device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, float* __restrict__ X,
+                                            float* __restrict__ Bs);
+
+template <int TW, int M, int L, int KIND>
+__global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __restrict__ work) {
+    // blocks [f * units, (f + 1) * units) relax field f of the set, unit by unit
+    using U = UnitLds<TW, M>;
+    __shared__ float X[U::NXS];
+    __shared__ float Bs[U::NBS];
+    const int f = blockIdx.x / S.units_per_field;
+    gs_unit_run<TW, M, L, KIND>(S.a[f], unit_of(S, work, blockIdx.x - f * S.units_per_field), X, Bs);
+}
+
+// One level launch of two solves pipelined together: blocks
+// [0, (count - 1) n_a) relax the diffuse fields 0 .. count - 2 of the set on the
+// units of work_a (n_a per field), the rest relax the pressure field count - 1
+// on the units of work_b.
+template <int TW, int M, int L>
+__global__ __launch_bounds__(64 * M) void k_gs_unit_mixed(SweepSet S, const int2* __restrict__ work_a, int n_a,
+                                                          const int2* __restrict__ work_b) {
+    using U = UnitLds<TW, M>;
+    __shared__ float X[U::NXS];
+    __shared__ float Bs[U::NBS];
+    const int b = blockIdx.x, nd = (S.count - 1) * n_a;
+    if (b < nd) {
+        const int f = b / n_a;
+        gs_unit_run<TW, M, L, SWEEP_DIFFUSE>(S.a[f], work_a[b - f * n_a], X, Bs);
+    } else {
+        gs_unit_run<TW, M, L, SWEEP_PRESSURE>(S.a[S.count - 1], work_b[b - nd], X, Bs);
+    }
+}
+
+template <int TW, int M, int L, int KIND>
+__device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, float* __restrict__ X,
+                                            float* __restrict__ Bs) {
+    using U = UnitLds<TW, M>;
+    constexpr int NT = 64 * M;
+    constexpr int XH = U::XH, XW = U::XW, XP = U::XP, BH = U::BH, BW = U::BW, BP = U::BP;
+    constexpr int NX = (XH * XW + NT - 1) / NT, NBX = (BH * BW + NT - 1) / NT;
     const int I = wk.x >> 16, J = wk.x & 0xffff, kap = wk.y;
     const int R = A.R, P = A.P, n = R - 2, K = A.K;
     const int k_lo = kap * M, k_hi = min(k_lo + M, K) - 1, k_top = k_lo + M - 1;
@@ -288,11 +323,22 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __re
             const int gi = min(max(gi_lo + idx / XW, 0), R - 1), gj = min(max(gj_lo + idx % XW, 0), R - 1);
             v[q] = x[(size_t)gi * P + gj];
         }
+        if (KIND == SWEEP_PRESSURE && A.du) {
+            // the divergence of (du, dv), src/fluid_solver_cpu.cpp:188-193
 #pragma unroll
-        for (int q = 0; q < NBX; ++q) {
-            const int idx = min(tid + q * NT, BH * BW - 1);
-            const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
-            vb[q] = A.b[(size_t)gi * P + gj];
+            for (int q = 0; q < NBX; ++q) {
+                const int idx = min(tid + q * NT, BH * BW - 1);
+                const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
+                const size_t o = (size_t)gi * P + gj;
+                vb[q] = A.neg_half_h * (A.du[o + 1] - A.du[o - 1] + A.dv[o + P] - A.dv[o - P]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NBX; ++q) {
+                const int idx = min(tid + q * NT, BH * BW - 1);
+                const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
+                vb[q] = A.b[(size_t)gi * P + gj];
+            }
         }
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
@@ -702,6 +748,8 @@ hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW) {
     plan->K = K;
     const int raw_levels = (K > 0) ? (plan->GI - 1) + (plan->GJ - 1) + 2 * (K - 1) + 1 : 0;
     plan->levels = 0;
+    plan->level_of = nullptr;  // tile plans are never pipelined
+    plan->host_work = nullptr;
     plan->work = nullptr;
     plan->host_offsets = nullptr;
     if (plan->GI >= (1 << 15) || plan->GJ >= (1 << 16)) return hipErrorInvalidValue;
@@ -736,6 +784,10 @@ hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW) {
 }
 
 hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int lag) {
+    return gs_unit_plan_build(plan, R, K, TW, M, lag, true);
+}
+
+hipError_t gs_unit_plan_build(GsPlan* plan, int R, int K, int TW, int M, int lag, bool upload) {
     const int n = R - 2;
     plan->TW = TW;
     plan->M = M;
@@ -746,6 +798,8 @@ hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int la
     plan->levels = 0;
     plan->work = nullptr;
     plan->host_offsets = nullptr;
+    plan->level_of = nullptr;
+    plan->host_work = nullptr;
     if (plan->GI >= (1 << 15) || plan->GJ >= (1 << 16)) return hipErrorInvalidValue;
     const int GK = (K + M - 1) / M;
     // units (I', J', kappa) holding at least one interior cell, by level I' + J' + kappa
@@ -761,18 +815,23 @@ hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int la
                 if (any) by_level[I + J + kap].push_back(make_int2((I << 16) | J, kap));
             }
     }
-    std::vector<int> offsets;
+    std::vector<int> offsets, level_of;
     std::vector<int2> work;
-    for (auto& lv : by_level) {
-        if (lv.empty()) continue;
+    for (size_t lv = 0; lv < by_level.size(); ++lv) {
+        if (by_level[lv].empty()) continue;
         offsets.push_back(static_cast<int>(work.size()));
-        work.insert(work.end(), lv.begin(), lv.end());
+        level_of.push_back(static_cast<int>(lv));
+        work.insert(work.end(), by_level[lv].begin(), by_level[lv].end());
     }
     plan->levels = static_cast<int>(offsets.size());
     offsets.push_back(static_cast<int>(work.size()));
     plan->host_offsets = new int[offsets.size()];
     std::copy(offsets.begin(), offsets.end(), plan->host_offsets);
-    if (work.empty()) return hipSuccess;
+    plan->level_of = new int[level_of.size() + 1];
+    std::copy(level_of.begin(), level_of.end(), plan->level_of);
+    plan->host_work = new int2[work.size() + 1];
+    std::copy(work.begin(), work.end(), plan->host_work);
+    if (work.empty() || !upload) return hipSuccess;
     hipError_t e = hipMalloc(&plan->work, sizeof(int2) * work.size());
     if (e != hipSuccess) return e;
     return hipMemcpy(plan->work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice);
@@ -781,8 +840,12 @@ hipError_t gs_unit_plan_create(GsPlan* plan, int R, int K, int TW, int M, int la
 void gs_plan_destroy(GsPlan* plan) {
     if (plan->work) (void)hipFree(plan->work);
     delete[] plan->host_offsets;
+    delete[] plan->level_of;
+    delete[] plan->host_work;
     plan->work = nullptr;
     plan->host_offsets = nullptr;
+    plan->level_of = nullptr;
+    plan->host_work = nullptr;
 }
 
 template <int TW, int M, int L>
@@ -804,6 +867,61 @@ bool launch_unit_lag(hipStream_t s, int lag, int n, const int2* w, const SweepSe
         case 8: launch_unit<TW, M, 8>(s, n, w, set); return true;
         default: return false;
     }
+}
+
+template <int TW, int M, int L>
+void launch_mixed(hipStream_t s, int n_a, const int2* wa, int n_b, const int2* wb, const SweepSet& set) {
+    hipLaunchKernelGGL((k_gs_unit_mixed<TW, M, L>), dim3((set.count - 1) * n_a + n_b), dim3(64 * M), 0, s, set, wa,
+                       n_a, wb);
+}
+
+template <int TW, int M>
+bool launch_mixed_lag(hipStream_t s, int lag, int n_a, const int2* wa, int n_b, const int2* wb, const SweepSet& set) {
+    switch (lag) {
+        case 1: launch_mixed<TW, M, 1>(s, n_a, wa, n_b, wb, set); return true;
+        case 4: launch_mixed<TW, M, 4>(s, n_a, wa, n_b, wb, set); return true;
+        case 8: launch_mixed<TW, M, 8>(s, n_a, wa, n_b, wb, set); return true;
+        default: return false;
+    }
+}
+
+bool launch_mixed_any(hipStream_t s, int TW, int M, int lag, int n_a, const int2* wa, int n_b, const int2* wb,
+                      const SweepSet& set) {
+    if (TW == 64) {
+        switch (M) {
+            case 2: return launch_mixed_lag<64, 2>(s, lag, n_a, wa, n_b, wb, set);
+            case 4: return launch_mixed_lag<64, 4>(s, lag, n_a, wa, n_b, wb, set);
+            case 8: return launch_mixed_lag<64, 8>(s, lag, n_a, wa, n_b, wb, set);
+            default: return false;
+        }
+    }
+    if (TW == 128) {
+        switch (M) {
+            case 4: return launch_mixed_lag<128, 4>(s, lag, n_a, wa, n_b, wb, set);
+            case 8: return launch_mixed_lag<128, 8>(s, lag, n_a, wa, n_b, wb, set);
+            default: return false;
+        }
+    }
+    return false;
+}
+
+bool launch_unit_any(hipStream_t s, int TW, int M, int lag, int n, const int2* w, const SweepSet& set) {
+    if (TW == 64) {
+        switch (M) {
+            case 2: return launch_unit_lag<64, 2>(s, lag, n, w, set);
+            case 4: return launch_unit_lag<64, 4>(s, lag, n, w, set);
+            case 8: return launch_unit_lag<64, 8>(s, lag, n, w, set);
+            default: return false;
+        }
+    }
+    if (TW == 128) {
+        switch (M) {
+            case 4: return launch_unit_lag<128, 4>(s, lag, n, w, set);
+            case 8: return launch_unit_lag<128, 8>(s, lag, n, w, set);
+            default: return false;
+        }
+    }
+    return false;
 }
 
 hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set) {
@@ -895,6 +1013,75 @@ hipError_t launch_gs_solve_implicit(hipStream_t s, int R, int K, int TW, int M, 
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+int gs_pipeline_offset(const GsPlan& pp, int R, int Kd, int TWd, int Md) {
+    // The diffuse's last writer of cell (i, j) -- interior, or a ring cell
+    // through its adjacent interior cell -- is the unit relaxing sweep Kd - 1
+    // there: skewed (i + Kd - 1, j + Kd - 1), so level
+    // (i + Kd - 2) / 64 + (j + Kd - 2) / TWd + (Kd - 1) / Md, non-decreasing in
+    // i and j.  A pressure unit reads the velocity one cell beyond the
+    // divergence region it loads (rows up to 64 I' - k_lo + 63, columns up to
+    // TW J' - k_lo + TW - 1, clamped into the interior); its last writer is at
+    // that region's far corner.
+    const int n = R - 2;
+    int off = 0;
+    for (int L = 0; L < pp.levels; ++L)
+        for (int e = pp.host_offsets[L]; e < pp.host_offsets[L + 1]; ++e) {
+            const int2 w = pp.host_work[e];
+            const int I = w.x >> 16, J = w.x & 0xffff, k_lo = w.y * pp.M;
+            const int ib = std::min(GS_ROWS * I - k_lo + GS_ROWS - 1, n) + 1;
+            const int jb = std::min(pp.TW * J - k_lo + pp.TW - 1, n) + 1;
+            const int ic = std::min(std::max(ib, 1), n), jc = std::min(std::max(jb, 1), n);
+            const int last_writer = (ic + Kd - 2) / GS_ROWS + (jc + Kd - 2) / TWd + (Kd - 1) / Md;
+            off = std::max(off, last_writer + 1 - pp.level_of[L]);
+        }
+    return off;
+}
+
+hipError_t launch_gs_pipelined(hipStream_t s, const GsPlan& pd, const SweepSet& diffuse, const GsPlan& pp,
+                               const SweepArgs& p, int offset) {
+    if (pd.M == 0 || pp.M == 0 || pd.TW != pp.TW || pd.M != pp.M || pd.lag != pp.lag || !pd.level_of ||
+        !pp.level_of || offset < 0 || pd.levels < 1 || pp.levels < 1)
+        return hipErrorInvalidValue;
+    if (diffuse.count < 1 || diffuse.count > SweepSet::kMax - 1 || p.kind != SWEEP_PRESSURE || p.K != pp.K)
+        return hipErrorInvalidValue;
+    for (int f = 0; f < diffuse.count; ++f)
+        if (diffuse.a[f].kind != SWEEP_DIFFUSE || diffuse.a[f].K != pd.K || diffuse.a[f].R != p.R)
+            return hipErrorInvalidValue;
+    SweepSet mixed = diffuse;
+    mixed.a[diffuse.count] = p;
+    mixed.count = diffuse.count + 1;
+    SweepSet pset{};
+    pset.a[0] = p;
+    pset.count = 1;
+    const int last = std::max(pd.level_of[pd.levels - 1], pp.level_of[pp.levels - 1] + offset);
+    int ld = 0, lp = 0;
+    for (int t = 0; t <= last; ++t) {
+        int na = 0, nb = 0;
+        const int2 *wa = nullptr, *wb = nullptr;
+        if (ld < pd.levels && pd.level_of[ld] == t) {
+            na = pd.host_offsets[ld + 1] - pd.host_offsets[ld];
+            wa = pd.work + pd.host_offsets[ld];
+            ++ld;
+        }
+        if (lp < pp.levels && pp.level_of[lp] + offset == t) {
+            nb = pp.host_offsets[lp + 1] - pp.host_offsets[lp];
+            wb = pp.work + pp.host_offsets[lp];
+            ++lp;
+        }
+        bool ok = true;
+        if (na > 0 && nb > 0)
+            ok = launch_mixed_any(s, pd.TW, pd.M, pd.lag, na, wa, nb, wb, mixed);
+        else if (na > 0)
+            ok = launch_unit_any(s, pd.TW, pd.M, pd.lag, na, wa, diffuse);
+        else if (nb > 0)
+            ok = launch_unit_any(s, pp.TW, pp.M, pp.lag, nb, wb, pset);
+        if (!ok) return hipErrorInvalidValue;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return (ld == pd.levels && lp == pp.levels) ? hipSuccess : hipErrorInvalidValue;
 }
 
 hipError_t launch_gs_solve(hipStream_t s, const GsPlan& plan, const SweepArgs& a) {

@@ -98,6 +98,7 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_stream": (_p, [H]),
         "fs2d_hip_time_pressure_sweeps": (_i, [H, _i, ctypes.POINTER(_f)]),
         "fs2d_hip_time_diffuse_sweeps": (_i, [H, _f, _f, _f, _i, ctypes.POINTER(_f)]),
+        "fs2d_gs_pipeline_offset": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_i)]),
         "fs2d_hip_gs_policy": (_i, [H, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
         "fs2d_hip_launches_per_step": (_i, [H, ctypes.POINTER(_i)]),
         "fs2d_hip_last_error": (ctypes.c_char_p, [H]),
@@ -294,7 +295,7 @@ EXPORTED_SYMBOLS = (
     "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
     "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
     "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",
-    "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_time_diffuse_sweeps", "fs2d_hip_gs_policy", "fs2d_hip_launches_per_step",
+    "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_time_diffuse_sweeps", "fs2d_hip_gs_policy", "fs2d_gs_pipeline_offset", "fs2d_hip_launches_per_step",
     "fs2d_hip_last_error", "fs2d_hip_abi_version",
     "fs2d_hip_render_density", "fs2d_hip_render_density_device", "fs2d_hip_render_velocity",
     "fs2d_hip_render_velocity_device", "fs2d_hip_time_render", "fs2d_hip_project_noalloc",
@@ -411,3 +412,16 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
     if rc != 0:
         raise Fs2dError(f"fs2d_hip_copy_bandwidth failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
     return float(g.value)
+
+
+def gs_pipeline_offset(rows: int, diffuse_iterations: int, pressure_iterations: int, tile_cols: int,
+                       sweeps_per_unit: int) -> int:
+    """Levels a Gauss-Seidel step's first pressure solve trails its diffuses by
+    when pipelined (host computation, no device call)."""
+    L = library()
+    off = _i()
+    rc = L.fs2d_gs_pipeline_offset(rows, diffuse_iterations, pressure_iterations, tile_cols, sweeps_per_unit,
+                                   ctypes.byref(off))
+    if rc != 0:
+        raise Fs2dError(f"fs2d_gs_pipeline_offset failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
+    return off.value

@@ -92,7 +92,10 @@ struct fs2d_handle;
 int fs2d_hip_project_seeded(struct fs2d_handle* h, fs2d_net* net, int seed, float tol, int max_sweeps, int check,
                             int fallback, int* sweeps_run, double* rel_residual, int* fell_back);
 /* Relative compact Poisson residual ||d - L p||_2 / ||d||_2 of the handle's
- * current divergence and pressure fields (FP64 sums). */
+ * stored divergence and pressure fields (FP64 sums): those of the last
+ * fs2d_hip_stage_project or seeded projection.  (A Gauss-Seidel step's
+ * projections compute the divergence where the pressure solve loads it and
+ * store none.) */
 int fs2d_hip_pressure_residual(struct fs2d_handle* h, double* rel_residual);
 
 #ifdef __cplusplus

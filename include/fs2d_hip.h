@@ -161,6 +161,13 @@ int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float d
  * one-sweep tile kernel), lag, and level launches per solve.  Chooses it first
  * if no solve of that kind has run (see gs_tile_cols, gs_block, gs_lag). */
 int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels);
+/* Host only (no device call): the levels by which a Gauss-Seidel step's first
+ * pressure solve trails its diffuses when both run in the same level launches
+ * (tile_cols x 64 tiles, sweeps_per_unit sweeps per unit, both solves one
+ * policy).  Every pressure unit then runs after the diffuse units that write
+ * the velocity its divergence reads. */
+int fs2d_gs_pipeline_offset(int rows, int diffuse_iterations, int pressure_iterations, int tile_cols,
+                            int sweeps_per_unit, int* offset);
 /* On-device copy bandwidth: a float4 copy kernel over `bytes` (read + write
  * counted), timed with HIP events over `reps` copies; the achievable HBM
  * figure beside the 8 TB/s peak. */

@@ -692,3 +692,61 @@ def jacobi_wave_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, TY
         zero = False
         done += m
     return src
+
+
+# ---------------------------------------------------------------------------
+# Two solves pipelined in the same level launches: the pressure solve of a
+# projection (right-hand side the divergence of the diffused velocity, computed
+# where each unit loads it) trailing the diffuse by `offset` levels.
+def gs_unit_writes(R, I, J, kap, K, TW, M):
+    """Cells (interior and ring) a unit of the diffuse plan writes back: every
+    cell it relaxed, and for a unit meeting the ring the ring beside its edge
+    cells (corners beside corner cells)."""
+    n = R - 2
+    k_lo, k_hi = kap * M, min(kap * M + M, K) - 1
+    k_top = k_lo + M - 1
+    gi = np.arange(ROWS * I - k_top, ROWS * I - k_top + ROWS + 1 + M)[:, None]
+    gj = np.arange(TW * J - k_top, TW * J - k_top + TW + 1 + M)[None, :]
+    lo = np.maximum(k_lo, np.maximum(ROWS * I + 1 - gi, TW * J + 1 - gj))
+    hi = np.minimum(k_hi, np.minimum(ROWS * I + ROWS - gi, TW * J + TW - gj))
+    relaxed = (gi >= 1) & (gi <= n) & (gj >= 1) & (gj <= n) & (lo <= hi)
+    ii, jj = np.nonzero(relaxed)
+    cells = set(zip((ii + gi[0, 0]).tolist(), (jj + gj[0, 0]).tolist()))
+    ring = set()
+    for (i, j) in cells:
+        if i == 1:
+            ring.add((0, j))
+        if i == n:
+            ring.add((R - 1, j))
+        if j == 1:
+            ring.add((i, 0))
+        if j == n:
+            ring.add((i, R - 1))
+        if i in (1, n) and j in (1, n):
+            ring.add((0 if i == 1 else R - 1, 0 if j == 1 else R - 1))
+    return cells | ring
+
+
+def gs_pipeline_min_offset(R, Kd, Kp, TW, M):
+    """Brute force: the smallest offset such that every pressure unit runs at a
+    later level than every diffuse unit that writes a velocity cell its
+    divergence reads (the four neighbours of each right-hand-side cell it
+    loads, clamped into the interior as the kernel clamps them)."""
+    n = R - 2
+    last = np.full((R, R), -1, dtype=np.int64)
+    for units in gs_unit_plan(R, Kd, TW, M):
+        for (I, J, kap) in units:
+            lvl = I + J + kap
+            for (i, j) in gs_unit_writes(R, I, J, kap, Kd, TW, M):
+                last[i, j] = max(last[i, j], lvl)
+    need = 0
+    for units in gs_unit_plan(R, Kp, TW, M):
+        for (I, J, kap) in units:
+            k_top = kap * M + M - 1
+            bi = np.clip(np.arange(ROWS * I - k_top + 1, ROWS * I - k_top + ROWS + M), 1, n)
+            bj = np.clip(np.arange(TW * J - k_top + 1, TW * J - k_top + TW + M), 1, n)
+            # the four neighbours: (i +- 1, j) and (i, j +- 1) of the region
+            m = max(last[np.ix_(np.unique(np.concatenate([bi - 1, bi + 1])), np.unique(bj))].max(),
+                    last[np.ix_(np.unique(bi), np.unique(np.concatenate([bj - 1, bj + 1])))].max())
+            need = max(need, int(m) + 1 - (I + J + kap))
+    return need

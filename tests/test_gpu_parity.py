@@ -350,7 +350,14 @@ def test_launch_policy_and_launch_count(gsb, tw, lag):
         assert q["lag"] == lag if lag else q["lag"] in (4, 8), q
     d, p = (len(gs_unit_plan(R, K, q["tile_cols"], q["sweeps_per_unit"])) for q in pol)
     assert (pol[0]["levels"], pol[1]["levels"]) == (d, p)
-    assert launches == 3 + d + scatter_launches(R) + 2 * (2 + p) + 1, (d, p, pol)
+    # the first projection's pressure solve runs in the diffuses' level launches
+    # (the diffuse's policy), `off` levels behind; the second one on its own
+    tw, m = pol[0]["tile_cols"], pol[0]["sweeps_per_unit"]
+    off = fs2d_amd.gs_pipeline_offset(R, K, K, tw, m)
+    lv = {u[0][0] + u[0][1] + u[0][2] for u in gs_unit_plan(R, K, tw, m)}
+    joint = len(lv | {x + off for x in lv})
+    # add x3, fill p, joint levels, scatter, gradient + fold, gather, fill p, p levels, gradient + fold
+    assert launches == 3 + 1 + joint + scatter_launches(R) + 2 + 1 + 1 + p + 2, (d, p, joint, pol)
 
 
 # ------------------------------------------------ the reference's own outputs

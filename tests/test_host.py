@@ -307,3 +307,23 @@ def test_vortex_state_deterministic_and_divergence_free():
     inner = (slice(3, -3), slice(3, -3))
     assert np.abs(du + dv)[inner].max() < 0.02 * (np.abs(du) + np.abs(dv))[inner].max()
     assert 0.0 <= rho.min() and rho.max() <= 1.0
+
+
+@pytest.mark.parametrize("R,Kd,Kp,TW,M", [(34, 8, 8, 64, 4), (66, 20, 20, 64, 8), (130, 20, 13, 128, 8),
+                                          (130, 13, 20, 64, 4), (200, 40, 40, 128, 8), (99, 7, 9, 64, 2),
+                                          (258, 20, 20, 128, 4)])
+def test_pipeline_offset_is_the_brute_force_minimum(R, Kd, Kp, TW, M):
+    """The level offset the library pipelines a step's first pressure solve by
+    (a closed form over the pressure plan) equals the smallest offset that a
+    brute-force model allows: every pressure unit after every diffuse unit
+    that writes (interior or ring) a velocity cell its divergence reads."""
+    from schedule_models import gs_pipeline_min_offset
+    assert fs2d_amd.gs_pipeline_offset(R, Kd, Kp, TW, M) == gs_pipeline_min_offset(R, Kd, Kp, TW, M)
+
+
+def test_pipeline_offset_at_the_workload_sizes():
+    """4096^2 and 8192^2 with K = 80 (128-column units of 8 sweeps): 13 levels;
+    1024^2 with K = 40: 7 -- against 107, 203 and 55 levels per solve."""
+    assert fs2d_amd.gs_pipeline_offset(4098, 80, 80, 128, 8) == 13
+    assert fs2d_amd.gs_pipeline_offset(8194, 80, 80, 128, 8) == 13
+    assert fs2d_amd.gs_pipeline_offset(1026, 40, 40, 128, 8) == 7
