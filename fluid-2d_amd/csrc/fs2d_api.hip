@@ -967,12 +967,27 @@ static int fs2d_hip_time_pressure_sweeps_impl(fs2d_handle* h, int reps, float* m
     if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_pressure_sweeps: bad argument");
     const Scalars s = scalars(h->R, 1.0f);
     const int K = h->prm.pressure_iterations;
+    const GsPlan* plan = nullptr;
+    if (K > 0 && h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h))
+        FS2D_HIP(h, gs_plan_for(h, K, SWEEP_PRESSURE, &plan));
+    int rc;
+    auto once = [&]() -> int {
+        if (plan && plan->M > 0) {  // as a step's second projection: divergence computed on load
+            FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
+            SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 0);
+            A.du = h->u();
+            A.dv = h->v();
+            A.neg_half_h = s.neg_half_h;
+            FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
+            return FS2D_OK;
+        }
+        return linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1);
+    };
     FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h, nullptr));
-    int rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1);  // warm
-    if (rc) return rc;
+    if ((rc = once())) return rc;  // warm
     FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
     for (int i = 0; i < reps; ++i)
-        if ((rc = linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1))) return rc;
+        if ((rc = once())) return rc;
     FS2D_HIP(h, hipEventRecord(h->ev1, h->stream));
     FS2D_HIP(h, hipEventSynchronize(h->ev1));
     float total = 0.f;
@@ -983,6 +998,58 @@ static int fs2d_hip_time_pressure_sweeps_impl(fs2d_handle* h, int reps, float* m
 
 int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms) {
     return guarded(h, "fs2d_hip_time_pressure_sweeps", [&] { return fs2d_hip_time_pressure_sweeps_impl(h, reps, ms); });
+}
+
+static int fs2d_hip_time_joint_sweeps_impl(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms) {
+    if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_joint_sweeps: bad argument");
+    const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
+    const int R = h->R, P = h->P;
+    if (h->prm.sweep_order != FS2D_ORDER_GS_LEX || !gs_units(h) || Kd <= 0 || Kp <= 0)
+        return fail(h, FS2D_ERR_STATE, "time_joint_sweeps: Gauss-Seidel units with diffuse and pressure sweeps only");
+    const GsPlan* plan = nullptr;
+    FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
+    if (plan->M <= 0) return fail(h, FS2D_ERR_STATE, "time_joint_sweeps: Gauss-Seidel units only");
+    const GsPolicy pol{plan->TW, plan->M, plan->lag};
+    const GsPlan* pplan = nullptr;
+    FS2D_HIP(h, gs_plan_of(h, Kp, pol, &pplan));
+    const int offset = pipeline_offset_for(h, Kd, *pplan, pol);
+    // iterates and right-hand sides are copies of the state, which stays untouched;
+    // the pressure iterate is the (otherwise unused) scratch field
+    const size_t n4 = h->field_bytes / 16;
+    const int pairs[6][2] = {{F_X0, F_RHO}, {F_X0U, F_U}, {F_X0V, F_V}, {F_T1, F_RHO}, {F_UN, F_U}, {F_VN, F_V}};
+    for (const auto& pr : pairs)
+        FS2D_HIP(h, launch_copy4(h->stream, reinterpret_cast<float4*>(h->f[pr[0]]),
+                                 reinterpret_cast<const float4*>(h->f[pr[1]]), n4));
+    const float ad = dt * cells(R) * diff, av = dt * cells(R) * visc;
+    SweepSet set{};
+    set.count = 3;
+    set.a[0] = sweep_args(h->f[F_T1], h->f[F_X0], R, P, SWEEP_DIFFUSE, ad, 1.f + 4.f * ad, BND_CONT, Kd, 0);
+    set.a[1] = sweep_args(h->f[F_UN], h->f[F_X0U], R, P, SWEEP_DIFFUSE, av, 1.f + 4.f * av, BND_OPP_H, Kd, 0);
+    set.a[2] = sweep_args(h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, 1.f + 4.f * av, BND_OPP_V, Kd, 0);
+    SweepArgs A = sweep_args(h->f[F_SCRATCH], h->f[F_D], R, P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, Kp, 0);
+    A.du = h->f[F_UN];
+    A.dv = h->f[F_VN];
+    A.neg_half_h = scalars(R, 1.0f).neg_half_h;
+    auto once = [&]() -> int {
+        FS2D_HIP(h, launch_fill(h->stream, h->f[F_SCRATCH], 0u, h->field_bytes / 4));
+        FS2D_HIP(h, launch_gs_pipelined(h->stream, *plan, set, *pplan, A, offset));
+        return FS2D_OK;
+    };
+    int rc = once();  // warm
+    if (rc) return rc;
+    FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < reps; ++i)
+        if ((rc = once())) return rc;
+    FS2D_HIP(h, hipEventRecord(h->ev1, h->stream));
+    FS2D_HIP(h, hipEventSynchronize(h->ev1));
+    float total = 0.f;
+    FS2D_HIP(h, hipEventElapsedTime(&total, h->ev0, h->ev1));
+    *ms = total / reps;
+    return FS2D_OK;
+}
+
+int fs2d_hip_time_joint_sweeps(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms) {
+    return guarded(h, "fs2d_hip_time_joint_sweeps", [&] { return fs2d_hip_time_joint_sweeps_impl(h, diff, visc, dt, reps, ms); });
 }
 
 static int fs2d_hip_time_diffuse_sweeps_impl(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms) {
@@ -1033,8 +1100,27 @@ int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float d
 }
 
 static int fs2d_hip_gs_policy_impl(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels) {
-    if (!h || (kind != 0 && kind != 1)) return fail(h, FS2D_ERR_ARGUMENT, "gs_policy: bad argument");
+    if (!h || kind < 0 || kind > 2) return fail(h, FS2D_ERR_ARGUMENT, "gs_policy: bad argument");
     if (h->prm.sweep_order != FS2D_ORDER_GS_LEX) return fail(h, FS2D_ERR_STATE, "gs_policy: Gauss-Seidel order only");
+    if (kind == 2) {  // the joint level launches of a step (the diffuse's policy)
+        const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
+        const GsPlan* dplan = nullptr;
+        if (Kd <= 0 || Kp <= 0 || !gs_units(h)) return fail(h, FS2D_ERR_STATE, "gs_policy: no joint launches");
+        FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &dplan));
+        if (dplan->M <= 0) return fail(h, FS2D_ERR_STATE, "gs_policy: no joint launches");
+        const GsPolicy pol{dplan->TW, dplan->M, dplan->lag};
+        const GsPlan* pplan = nullptr;
+        FS2D_HIP(h, gs_plan_of(h, Kp, pol, &pplan));
+        const int off = pipeline_offset_for(h, Kd, *pplan, pol);
+        std::vector<int> lv(dplan->level_of, dplan->level_of + dplan->levels);
+        for (int l = 0; l < pplan->levels; ++l) lv.push_back(pplan->level_of[l] + off);
+        std::sort(lv.begin(), lv.end());
+        if (tile_cols) *tile_cols = pol.tw;
+        if (sweeps_per_unit) *sweeps_per_unit = pol.M;
+        if (lag) *lag = pol.lag;
+        if (levels) *levels = static_cast<int>(std::unique(lv.begin(), lv.end()) - lv.begin());
+        return FS2D_OK;
+    }
     const int K = kind == 0 ? h->prm.diffuse_iterations : h->prm.pressure_iterations;
     if (K <= 0) return fail(h, FS2D_ERR_STATE, "gs_policy: no sweeps of that kind");
     const int sk = kind == 0 ? SWEEP_DIFFUSE : SWEEP_PRESSURE;

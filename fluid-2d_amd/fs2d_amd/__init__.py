@@ -98,6 +98,7 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_stream": (_p, [H]),
         "fs2d_hip_time_pressure_sweeps": (_i, [H, _i, ctypes.POINTER(_f)]),
         "fs2d_hip_time_diffuse_sweeps": (_i, [H, _f, _f, _f, _i, ctypes.POINTER(_f)]),
+        "fs2d_hip_time_joint_sweeps": (_i, [H, _f, _f, _f, _i, ctypes.POINTER(_f)]),
         "fs2d_gs_pipeline_offset": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_i)]),
         "fs2d_hip_gs_policy": (_i, [H, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
         "fs2d_hip_launches_per_step": (_i, [H, ctypes.POINTER(_i)]),
@@ -241,6 +242,13 @@ class Solver:
         self._ck(self._L.fs2d_hip_time_pressure_sweeps(self._h, reps, ctypes.byref(ms)))
         return float(ms.value)
 
+    def time_joint_sweeps(self, diffusion: float, viscosity: float, dt: float, reps: int) -> float:
+        """Milliseconds of a Gauss-Seidel step's joint level launches (three
+        diffuses and the first pressure solve, pipelined), on copies of the state."""
+        ms = _f()
+        self._ck(self._L.fs2d_hip_time_joint_sweeps(self._h, diffusion, viscosity, dt, reps, ctypes.byref(ms)))
+        return float(ms.value)
+
     def gs_policy(self, kind: int) -> dict:
         """Gauss-Seidel launch policy of the diffuse (0) or pressure (1) solves."""
         tw, m, lag, lv = _i(), _i(), _i(), _i()
@@ -295,7 +303,7 @@ EXPORTED_SYMBOLS = (
     "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
     "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
     "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",
-    "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_time_diffuse_sweeps", "fs2d_hip_gs_policy", "fs2d_gs_pipeline_offset", "fs2d_hip_launches_per_step",
+    "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_time_diffuse_sweeps", "fs2d_hip_time_joint_sweeps", "fs2d_hip_gs_policy", "fs2d_gs_pipeline_offset", "fs2d_hip_launches_per_step",
     "fs2d_hip_last_error", "fs2d_hip_abi_version",
     "fs2d_hip_render_density", "fs2d_hip_render_density_device", "fs2d_hip_render_velocity",
     "fs2d_hip_render_velocity_device", "fs2d_hip_time_render", "fs2d_hip_project_noalloc",

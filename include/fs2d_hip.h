@@ -146,9 +146,11 @@ int fs2d_hip_download_pressure(fs2d_handle* h, float* p);
 /* The handle's HIP stream (a hipStream_t), for callers that record events. */
 void* fs2d_hip_stream(fs2d_handle* h);
 /* Enqueue only the relaxation sweeps of one pressure solve (K_p sweeps on the
- * current divergence) `reps` times and return the mean time of one solve in
- * milliseconds, measured with HIP events on the handle's stream.  The state's
- * velocity is not modified. */
+ * current velocity's divergence; with Gauss-Seidel units as a step's second
+ * projection runs it, the divergence computed on load, the zero fill included)
+ * `reps` times and return the mean time of one solve in milliseconds,
+ * measured with HIP events on the handle's stream.  The state's velocity is
+ * not modified. */
 int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms_per_solve);
 /* Enqueue the three diffuse solves of one step (density, u, v: K_d sweeps
  * each, shared level launches in the Gauss-Seidel order as the step runs
@@ -156,10 +158,18 @@ int fs2d_hip_time_pressure_sweeps(fs2d_handle* h, int reps, float* ms_per_solve)
  * step's diffuses in milliseconds, measured with HIP events on the handle's
  * stream.  The state (density, velocity, pressure) is not modified. */
 int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms_per_solve);
+/* The Gauss-Seidel step's joint level launches -- the three diffuses and the
+ * first projection's pressure solve pipelined behind them (divergence of the
+ * diffused velocity computed on load) -- `reps` times on copies of the state:
+ * mean milliseconds per run (HIP events, the pressure iterate's zero fill
+ * included).  Gauss-Seidel units only. */
+int fs2d_hip_time_joint_sweeps(fs2d_handle* h, float diff, float visc, float dt, int reps, float* ms_per_solve);
 /* The Gauss-Seidel launch policy of the diffuse (kind 0, K_d sweeps) or
  * pressure (kind 1, K_p sweeps) solves: tile columns, sweeps per unit (1: the
  * one-sweep tile kernel), lag, and level launches per solve.  Chooses it first
- * if no solve of that kind has run (see gs_tile_cols, gs_block, gs_lag). */
+ * if no solve of that kind has run (see gs_tile_cols, gs_block, gs_lag).
+ * Kind 2: the step's joint launches (the diffuses with the first projection's
+ * pressure solve pipelined behind them, on the diffuse's policy). */
 int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels);
 /* Host only (no device call): the levels by which a Gauss-Seidel step's first
  * pressure solve trails its diffuses when both run in the same level launches

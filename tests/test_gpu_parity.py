@@ -487,8 +487,9 @@ def test_jacobi_autotuned_depth(jk, oracle_lib):
 
 @pytest.mark.parametrize("order", [GS, JAC])
 def test_solve_timers_leave_the_state(order, oracle_lib):
-    """The event-timed diffuse and pressure solves run on scratch fields: the
-    next step from the same state is still the oracle's step."""
+    """The event-timed diffuse, joint and pressure solves run on scratch fields
+    (the pressure timer only changes the pressure field): the next step from
+    the same state is still the oracle's step."""
     R = 130
     st = W.vortex_state(R, seed=9, n_vortices=16)
     z = np.zeros((R, R), np.float32)
@@ -498,6 +499,10 @@ def test_solve_timers_leave_the_state(order, oracle_lib):
         s.set_sources(None, None, None)
         ms_d = s.time_diffuse_sweeps(1e-4, 1e-5, 0.05, 3)
         assert [np.array_equal(a, b) for a, b in zip(s.download(), st)] == [True] * 3
+        if order == GS:
+            assert s.time_joint_sweeps(1e-4, 1e-5, 0.05, 3) > 0
+            assert s.time_pressure_sweeps(3) > 0
+            assert [np.array_equal(a, b) for a, b in zip(s.download(), st)] == [True] * 3
         s.step_n(1, 1e-4, 1e-5, 0.05)
         got = s.download()
     oracle_lib.solve(want[0], z, 1e-4, want[1], want[2], z, z, 1e-5, 0.05, 20, 20,
