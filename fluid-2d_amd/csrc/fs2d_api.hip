@@ -55,6 +55,7 @@ struct fs2d_handle {
     int gs_tw = 0, gs_block = 0, gs_lag = 0;
     std::map<std::pair<int, int>, GsPolicy> gs_policy;  // (kind, K) -> policy in use
     std::map<std::tuple<int, int, int, int, int>, int> pipe_offsets;  // (Kd, Kp, tw, M, lag) -> pipeline offset
+    std::map<std::pair<int, int>, GsPolicy> joint_policy;             // (Kd, Kp) -> policy of the joint launches
     int jacobi_kernel = FS2D_JACOBI_WAVE;
     std::map<std::pair<int, int>, int> jacobi_depth;  // (kind, K) -> sweeps per launch, when tuned
     bool pressure_zero = true;  // last projection had K_p == 0 (p is identically 0)
@@ -143,17 +144,9 @@ hipError_t gs_plan_of(fs2d_handle* h, int K, const GsPolicy& c, const GsPlan** o
 std::mutex g_policy_mu;
 std::map<std::tuple<int, int, int, int, int, int, int>, GsPolicy> g_policy_cache;
 
-hipError_t gs_policy_for(fs2d_handle* h, int K, int kind, GsPolicy* out) {
-    const auto hkey = std::make_pair(kind, K);
-    auto it = h->gs_policy.find(hkey);
-    if (it != h->gs_policy.end()) {
-        *out = it->second;
-        return hipSuccess;
-    }
-    GsPolicy def{};
-    def.M = h->gs_block == 1 ? 0 : (h->gs_block ? h->gs_block : 8);
-    def.tw = h->gs_tw ? h->gs_tw : ((kind == SWEEP_DIFFUSE && def.M != 2) ? 128 : 64);
-    def.lag = def.M == 0 ? 1 : (h->gs_lag ? h->gs_lag : 4);
+// The launch policies open to the tuner: tile columns {64, 128} x sweeps per
+// unit {4, 8} x lag {4, 8}, each parameter fixed where the handle fixes it.
+std::vector<GsPolicy> policy_candidates(const fs2d_handle* h) {
     std::vector<GsPolicy> cand;
     for (int tw : {64, 128})
         for (int M : {4, 8})
@@ -166,6 +159,21 @@ hipError_t gs_policy_for(fs2d_handle* h, int K, int kind, GsPolicy* out) {
                 for (const auto& d : cand) dup = dup || (d.tw == c.tw && d.M == c.M && d.lag == c.lag);
                 if (!dup) cand.push_back(c);
             }
+    return cand;
+}
+
+hipError_t gs_policy_for(fs2d_handle* h, int K, int kind, GsPolicy* out) {
+    const auto hkey = std::make_pair(kind, K);
+    auto it = h->gs_policy.find(hkey);
+    if (it != h->gs_policy.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    GsPolicy def{};
+    def.M = h->gs_block == 1 ? 0 : (h->gs_block ? h->gs_block : 8);
+    def.tw = h->gs_tw ? h->gs_tw : ((kind == SWEEP_DIFFUSE && def.M != 2) ? 128 : 64);
+    def.lag = def.M == 0 ? 1 : (h->gs_lag ? h->gs_lag : 4);
+    const std::vector<GsPolicy> cand = policy_candidates(h);
     if (cand.size() <= 1 || K <= 0) {
         *out = cand.size() == 1 ? cand[0] : def;
         h->gs_policy.emplace(hkey, *out);
@@ -405,6 +413,116 @@ int pipeline_offset_for(fs2d_handle* h, int Kd, const GsPlan& pp, const GsPolicy
     return off;
 }
 
+// The level launches that relax a step's three diffuses together with its
+// first pressure solve (launch_gs_pipelined): one launch policy for both kinds.
+struct JointPlans {
+    const GsPlan* d = nullptr;  // null: the step does not pipeline (no sweeps of a kind, or no units)
+    const GsPlan* p = nullptr;
+    int offset = 0;
+};
+
+// The joint launches' policy, chosen like gs_policy_for's but on what they
+// run: every candidate relaxes three zero-started scratch diffuse iterates and
+// a scratch pressure iterate whose right-hand side is computed on load from
+// two of them, pipelined at that policy's offset, timed with HIP events (best
+// of two after a warm-up); the fastest is kept, also process-wide for the
+// same (device, rows, Kd, Kp, fixed parameters).  Under capture the diffuse's
+// policy serves and is not kept.
+std::map<std::tuple<int, int, int, int, int, int, int>, GsPolicy> g_joint_cache;
+
+hipError_t joint_plans(fs2d_handle* h, int Kd, int Kp, JointPlans* out) {
+    *out = JointPlans{};
+    if (Kd <= 0 || Kp <= 0 || h->prm.sweep_order != FS2D_ORDER_GS_LEX || !gs_units(h)) return hipSuccess;
+    auto finish = [&](const GsPolicy& c) -> hipError_t {
+        hipError_t e = gs_plan_of(h, Kd, c, &out->d);
+        if (e == hipSuccess) e = gs_plan_of(h, Kp, c, &out->p);
+        if (e != hipSuccess) return *out = JointPlans{}, e;
+        out->offset = pipeline_offset_for(h, Kd, *out->p, c);
+        return hipSuccess;
+    };
+    const auto hkey = std::make_pair(Kd, Kp);
+    auto it = h->joint_policy.find(hkey);
+    if (it != h->joint_policy.end()) return finish(it->second);
+    GsPolicy def{};
+    hipError_t e = gs_policy_for(h, Kd, SWEEP_DIFFUSE, &def);
+    if (e != hipSuccess) return e;
+    if (def.M <= 0) return hipSuccess;
+    std::vector<GsPolicy> cand;
+    for (const GsPolicy& c : policy_candidates(h))
+        if (c.M > 0) cand.push_back(c);
+    if (cand.size() <= 1) {
+        const GsPolicy c = cand.empty() ? def : cand[0];
+        h->joint_policy.emplace(hkey, c);
+        return finish(c);
+    }
+    const auto gkey = std::make_tuple(h->device, h->R, Kd, Kp, h->gs_tw, h->gs_block, h->gs_lag);
+    {
+        std::lock_guard<std::mutex> lock(g_policy_mu);
+        auto g = g_joint_cache.find(gkey);
+        if (g != g_joint_cache.end()) {
+            h->joint_policy.emplace(hkey, g->second);
+            return finish(g->second);
+        }
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if ((e = hipStreamIsCapturing(h->stream, &cs)) != hipSuccess) return e;
+    if (cs != hipStreamCaptureStatusNone) return finish(def);
+    float* tmp[4] = {nullptr, nullptr, nullptr, nullptr};
+    auto release = [&]() {
+        for (float* q : tmp)
+            if (q) (void)hipFree(q);
+    };
+    for (float*& q : tmp)
+        if ((e = hipMalloc(&q, h->field_bytes)) != hipSuccess) return release(), e;
+    SweepSet set{};
+    set.count = 3;
+    const int modes[3] = {BND_CONT, BND_OPP_H, BND_OPP_V};
+    const int rhs[3] = {F_RHO, F_U, F_V};
+    for (int f = 0; f < 3; ++f)
+        set.a[f] = sweep_args(tmp[f], h->f[rhs[f]], h->R, h->P, SWEEP_DIFFUSE, 1.0f, 5.0f, modes[f], Kd, 0);
+    SweepArgs A = sweep_args(tmp[3], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, Kp, 0);
+    A.du = tmp[1];
+    A.dv = tmp[2];
+    A.neg_half_h = scalars(h->R, 1.0f).neg_half_h;
+    float best_ms = 0.f;
+    GsPolicy best = def;
+    for (size_t c = 0; c < cand.size(); ++c) {
+        if ((e = finish(cand[c])) != hipSuccess) return release(), e;
+        float t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            for (int f = 0; f < 4 && e == hipSuccess; ++f) e = launch_fill(h->stream, tmp[f], 0u, h->field_bytes / 4);
+            if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+            if (e == hipSuccess) e = launch_gs_pipelined(h->stream, *out->d, set, *out->p, A, out->offset);
+            if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+            if (e != hipSuccess) return release(), e;
+            if (rep == 1 || (rep > 1 && ms < t)) t = ms;
+        }
+        if (c == 0 || t < best_ms) {
+            best_ms = t;
+            best = cand[c];
+        }
+    }
+    release();
+    {
+        std::lock_guard<std::mutex> lock(g_policy_mu);
+        g_joint_cache.emplace(gkey, best);
+    }
+    h->joint_policy.emplace(hkey, best);
+    return finish(best);
+}
+
+// Distinct level launches of a joint run: the diffuse's levels and the
+// pressure's, shifted by the offset.
+int joint_levels(const JointPlans& j) {
+    std::vector<int> lv(j.d->level_of, j.d->level_of + j.d->levels);
+    for (int l = 0; l < j.p->levels; ++l) lv.push_back(j.p->level_of[l] + j.offset);
+    std::sort(lv.begin(), lv.end());
+    return static_cast<int>(std::unique(lv.begin(), lv.end()) - lv.begin());
+}
+
 // A projection whose pressure solve (Gauss-Seidel units) computes the
 // divergence of (u, v) where each unit loads its right-hand side, so no
 // divergence pass runs and none is stored; otherwise project().
@@ -449,23 +567,19 @@ int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
         set.a[0] = sweep_args(h->rho(), h->f[F_X0], R, P, SWEEP_DIFFUSE, ad, dd, BND_CONT, Kd, 0);
         set.a[1] = sweep_args(h->f[F_UN], h->f[F_X0U], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_H, Kd, 0);
         set.a[2] = sweep_args(h->f[F_VN], h->f[F_X0V], R, P, SWEEP_DIFFUSE, av, dv, BND_OPP_V, Kd, 0);
-        const GsPlan* plan = nullptr;
-        FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
-        if (Kp > 0 && plan->M > 0) {
+        JointPlans joint;
+        FS2D_HIP(h, joint_plans(h, Kd, Kp, &joint));
+        if (joint.d) {
             // The first projection pipelined behind the diffuses: its pressure
-            // solve (the diffuse's launch policy, right-hand side the divergence
-            // of the diffused velocity, computed where each unit loads it) runs in
-            // the same level launches, `offset` levels behind.
-            const GsPolicy pol{plan->TW, plan->M, plan->lag};
-            const GsPlan* pplan = nullptr;
-            FS2D_HIP(h, gs_plan_of(h, Kp, pol, &pplan));
-            const int offset = pipeline_offset_for(h, Kd, *pplan, pol);
+            // solve (right-hand side the divergence of the diffused velocity,
+            // computed where each unit loads it) runs in the same level
+            // launches, `offset` levels behind.
             FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
             SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], R, P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, Kp, 0);
             A.du = h->f[F_UN];
             A.dv = h->f[F_VN];
             A.neg_half_h = scalars(R, 1.0f).neg_half_h;
-            FS2D_HIP(h, launch_gs_pipelined(h->stream, *plan, set, *pplan, A, offset));
+            FS2D_HIP(h, launch_gs_pipelined(h->stream, *joint.d, set, *joint.p, A, joint.offset));
             if ((rc = advect_density(h, dt))) return rc;  // reads the pre-step u, v
             std::swap(h->u(), h->f[F_UN]);
             std::swap(h->v(), h->f[F_VN]);
@@ -476,6 +590,8 @@ int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
             if ((rc = advect_velocity(h, dt))) return rc;
             return project_onload(h, Kp);
         }
+        const GsPlan* plan = nullptr;
+        FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
         FS2D_HIP(h, launch_gs_solve_set(h->stream, *plan, set));
     }
     if ((rc = advect_density(h, dt))) return rc;  // reads the pre-step u, v
@@ -1006,13 +1122,9 @@ static int fs2d_hip_time_joint_sweeps_impl(fs2d_handle* h, float diff, float vis
     const int R = h->R, P = h->P;
     if (h->prm.sweep_order != FS2D_ORDER_GS_LEX || !gs_units(h) || Kd <= 0 || Kp <= 0)
         return fail(h, FS2D_ERR_STATE, "time_joint_sweeps: Gauss-Seidel units with diffuse and pressure sweeps only");
-    const GsPlan* plan = nullptr;
-    FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
-    if (plan->M <= 0) return fail(h, FS2D_ERR_STATE, "time_joint_sweeps: Gauss-Seidel units only");
-    const GsPolicy pol{plan->TW, plan->M, plan->lag};
-    const GsPlan* pplan = nullptr;
-    FS2D_HIP(h, gs_plan_of(h, Kp, pol, &pplan));
-    const int offset = pipeline_offset_for(h, Kd, *pplan, pol);
+    JointPlans joint;
+    FS2D_HIP(h, joint_plans(h, Kd, Kp, &joint));
+    if (!joint.d) return fail(h, FS2D_ERR_STATE, "time_joint_sweeps: Gauss-Seidel units only");
     // iterates and right-hand sides are copies of the state, which stays untouched;
     // the pressure iterate is the (otherwise unused) scratch field
     const size_t n4 = h->field_bytes / 16;
@@ -1032,7 +1144,7 @@ static int fs2d_hip_time_joint_sweeps_impl(fs2d_handle* h, float diff, float vis
     A.neg_half_h = scalars(R, 1.0f).neg_half_h;
     auto once = [&]() -> int {
         FS2D_HIP(h, launch_fill(h->stream, h->f[F_SCRATCH], 0u, h->field_bytes / 4));
-        FS2D_HIP(h, launch_gs_pipelined(h->stream, *plan, set, *pplan, A, offset));
+        FS2D_HIP(h, launch_gs_pipelined(h->stream, *joint.d, set, *joint.p, A, joint.offset));
         return FS2D_OK;
     };
     int rc = once();  // warm
@@ -1102,23 +1214,14 @@ int fs2d_hip_time_diffuse_sweeps(fs2d_handle* h, float diff, float visc, float d
 static int fs2d_hip_gs_policy_impl(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per_unit, int* lag, int* levels) {
     if (!h || kind < 0 || kind > 2) return fail(h, FS2D_ERR_ARGUMENT, "gs_policy: bad argument");
     if (h->prm.sweep_order != FS2D_ORDER_GS_LEX) return fail(h, FS2D_ERR_STATE, "gs_policy: Gauss-Seidel order only");
-    if (kind == 2) {  // the joint level launches of a step (the diffuse's policy)
-        const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
-        const GsPlan* dplan = nullptr;
-        if (Kd <= 0 || Kp <= 0 || !gs_units(h)) return fail(h, FS2D_ERR_STATE, "gs_policy: no joint launches");
-        FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &dplan));
-        if (dplan->M <= 0) return fail(h, FS2D_ERR_STATE, "gs_policy: no joint launches");
-        const GsPolicy pol{dplan->TW, dplan->M, dplan->lag};
-        const GsPlan* pplan = nullptr;
-        FS2D_HIP(h, gs_plan_of(h, Kp, pol, &pplan));
-        const int off = pipeline_offset_for(h, Kd, *pplan, pol);
-        std::vector<int> lv(dplan->level_of, dplan->level_of + dplan->levels);
-        for (int l = 0; l < pplan->levels; ++l) lv.push_back(pplan->level_of[l] + off);
-        std::sort(lv.begin(), lv.end());
-        if (tile_cols) *tile_cols = pol.tw;
-        if (sweeps_per_unit) *sweeps_per_unit = pol.M;
-        if (lag) *lag = pol.lag;
-        if (levels) *levels = static_cast<int>(std::unique(lv.begin(), lv.end()) - lv.begin());
+    if (kind == 2) {  // the joint level launches of a step
+        JointPlans joint;
+        FS2D_HIP(h, joint_plans(h, h->prm.diffuse_iterations, h->prm.pressure_iterations, &joint));
+        if (!joint.d) return fail(h, FS2D_ERR_STATE, "gs_policy: no joint launches");
+        if (tile_cols) *tile_cols = joint.d->TW;
+        if (sweeps_per_unit) *sweeps_per_unit = joint.d->M;
+        if (lag) *lag = joint.d->lag;
+        if (levels) *levels = joint_levels(joint);
         return FS2D_OK;
     }
     const int K = kind == 0 ? h->prm.diffuse_iterations : h->prm.pressure_iterations;
@@ -1202,19 +1305,12 @@ static int fs2d_hip_launches_per_step_impl(fs2d_handle* h, int* launches) {
     const int d = solve_launches(Kd, SWEEP_DIFFUSE), p = solve_launches(Kp, SWEEP_PRESSURE);
     if (d < 0 || p < 0) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
     const bool shared = h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h);
-    const GsPlan* dplan = nullptr;
-    if (shared && Kd > 0 && Kp > 0 && gs_plan_for(h, Kd, SWEEP_DIFFUSE, &dplan) == hipSuccess && dplan->M > 0) {
+    JointPlans joint;
+    if (joint_plans(h, Kd, Kp, &joint) != hipSuccess) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
+    if (joint.d) {
         // pipelined: add x3, fill p, the joint level launches, scatter, gradient +
         // fold, gather, then fill p, the second solve's levels, gradient + fold
-        const GsPolicy pol{dplan->TW, dplan->M, dplan->lag};
-        const GsPlan* pplan = nullptr;
-        if (gs_plan_of(h, Kp, pol, &pplan) != hipSuccess) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
-        const int off = pipeline_offset_for(h, Kd, *pplan, pol);
-        std::vector<int> lv(dplan->level_of, dplan->level_of + dplan->levels);
-        for (int l = 0; l < pplan->levels; ++l) lv.push_back(pplan->level_of[l] + off);
-        std::sort(lv.begin(), lv.end());
-        const int joint = static_cast<int>(std::unique(lv.begin(), lv.end()) - lv.begin());
-        *launches = 3 + 1 + joint + scatter_launches(h->R) + 2 + 1 + 1 + p + 2;
+        *launches = 3 + 1 + joint_levels(joint) + scatter_launches(h->R) + 2 + 1 + 1 + p + 2;
         return FS2D_OK;
     }
     // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather

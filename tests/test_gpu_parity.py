@@ -342,20 +342,22 @@ def test_launch_policy_and_launch_count(gsb, tw, lag):
     R, K = 258, 20
     with fs2d_amd.Solver(R, diffuse_iterations=K, pressure_iterations=K, gs_block=gsb, gs_tile_cols=tw,
                          gs_lag=lag) as s:
-        pol = [s.gs_policy(kind) for kind in (0, 1)]
+        pol = [s.gs_policy(kind) for kind in (0, 1, 2)]
         launches = s.launches_per_step()
     for q in pol:
         assert q["tile_cols"] == tw if tw else q["tile_cols"] in ((64,) if gsb == 2 else (64, 128)), q
         assert q["sweeps_per_unit"] == gsb if gsb else q["sweeps_per_unit"] in (4, 8), q
         assert q["lag"] == lag if lag else q["lag"] in (4, 8), q
-    d, p = (len(gs_unit_plan(R, K, q["tile_cols"], q["sweeps_per_unit"])) for q in pol)
+    d, p = (len(gs_unit_plan(R, K, q["tile_cols"], q["sweeps_per_unit"])) for q in pol[:2])
     assert (pol[0]["levels"], pol[1]["levels"]) == (d, p)
     # the first projection's pressure solve runs in the diffuses' level launches
-    # (the diffuse's policy), `off` levels behind; the second one on its own
-    tw, m = pol[0]["tile_cols"], pol[0]["sweeps_per_unit"]
+    # (the joint policy, tuned on its own), `off` levels behind; the second one
+    # on its own
+    tw, m = pol[2]["tile_cols"], pol[2]["sweeps_per_unit"]
     off = fs2d_amd.gs_pipeline_offset(R, K, K, tw, m)
     lv = {u[0][0] + u[0][1] + u[0][2] for u in gs_unit_plan(R, K, tw, m)}
     joint = len(lv | {x + off for x in lv})
+    assert pol[2]["levels"] == joint, (pol[2], joint)
     # add x3, fill p, joint levels, scatter, gradient + fold, gather, fill p, p levels, gradient + fold
     assert launches == 3 + 1 + joint + scatter_launches(R) + 2 + 1 + 1 + p + 2, (d, p, joint, pol)
 
