@@ -191,6 +191,111 @@ def poisson_dct(d: np.ndarray) -> np.ndarray:
     return idctn(ph, type=2, norm="ortho")
 
 
+def jacobi_spectrum(n: int):
+    """Eigenvalues on the DCT-II modes (k, l) of the pressure equation with the
+    continuous ring: Jacobi's iteration matrix (pN + pS + pW + pE) / 4 has
+    mu = (cos(pi k / n) + cos(pi l / n)) / 2 and the operator L = 4 - 4 mu."""
+    c = np.cos(np.pi * np.arange(n) / n)
+    mu = 0.5 * (c[:, None] + c[None, :])
+    return mu, 4.0 - 4.0 * mu
+
+
+def jacobi_residual_model(d: np.ndarray, p0: np.ndarray, sweeps, keep: np.ndarray | None = None):
+    """Relative compact residuals ||d - L p_k|| / ||d|| of Jacobi from p0 after
+    each count in `sweeps`, exactly (float64, no rounding): the error
+    e = p* - p0 (p* = poisson_dct(d)) decays as mu^k per DCT mode and the
+    residual is L e plus d's mean, which no p can cancel (L p sums to zero
+    under the continuous ring).  `keep` (bool per mode) restricts the starting
+    error to those modes: the start then equals p* in every other mode."""
+    from scipy.fft import dctn
+
+    n = d.shape[0]
+    mu, lam = jacobi_spectrum(n)
+    eh = dctn(poisson_dct(d) - p0, type=2, norm="ortho")
+    if keep is not None:
+        eh = np.where(keep, eh, 0.0)
+    a = (lam * eh) ** 2
+    a[0, 0] = 0.0
+    mean2 = float(n * n) * float(np.mean(d)) ** 2
+    m2 = mu * mu
+    nd = float(np.linalg.norm(d))
+    out = []
+    for k in np.atleast_1d(sweeps):
+        out.append(np.sqrt(mean2 + float(np.sum(a * m2 ** int(k)))) / nd)
+    return np.asarray(out)
+
+
+def sweeps_to_reach(d: np.ndarray, p0: np.ndarray, target: float, keep: np.ndarray | None = None,
+                    limit: int = 1 << 22) -> int:
+    """Fewest Jacobi sweeps from p0 whose modelled relative residual is at most
+    `target` (the residual never grows: |mu| <= 1), or -1 past `limit`."""
+    from scipy.fft import dctn
+
+    n = d.shape[0]
+    mu, lam = jacobi_spectrum(n)
+    eh = dctn(poisson_dct(d) - p0, type=2, norm="ortho")
+    if keep is not None:
+        eh = np.where(keep, eh, 0.0)
+    a = ((lam * eh) ** 2).ravel()
+    a[0] = 0.0
+    nz = a > 0
+    a, lm = a[nz], np.log(np.maximum((mu * mu).ravel()[nz], 1e-300))
+    budget = (target * float(np.linalg.norm(d))) ** 2 - float(n * n) * float(np.mean(d)) ** 2
+
+    def ok(k: int) -> bool:
+        return float(np.sum(a * np.exp(lm * k))) <= budget
+
+    if budget < 0:
+        return -1
+    if ok(0):
+        return 0
+    hi = 1
+    while not ok(hi):
+        hi *= 2
+        if hi > limit:
+            return -1
+    lo = hi // 2
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if ok(mid):
+            hi = mid
+        else:
+            lo = mid
+    return hi
+
+
+# Receptive field of the network in cells: two 3x3 convolutions per level
+# (16/32/64/128 channels), 2 x 2 pools down, bilinear x2 up, two convolutions per
+# decoder level and the output convolution give each output cell a 112 x 112
+# window of its inputs.
+RECEPTIVE_FIELD = 112
+
+
+def local_seed_bound(d: np.ndarray, p_prev: np.ndarray, budget: int, wavelengths=(RECEPTIVE_FIELD,)):
+    """What any seed computed from a window of the inputs can gain over plain
+    Jacobi from the previous pressure, on one projection.  The target is the
+    residual Jacobi reaches from p_prev in `budget` sweeps; for each window
+    side W, the idealised local seed equals the exact pressure p* in every
+    DCT mode of wavelength 2 n / |(k, l)| shorter than W and p_prev in the
+    others (the modes no window of side W resolves), and the result is the
+    sweeps it still needs to reach the target.  Returns a dict of the target,
+    the sweeps from p_prev and, per W, the sweeps and their ratio (the best
+    speedup in sweeps, None where the seed alone meets the target; a
+    network's own forward time only lowers it)."""
+    n = d.shape[0]
+    target = float(jacobi_residual_model(d, p_prev, [budget])[0])
+    base = sweeps_to_reach(d, p_prev, target)
+    k = np.arange(n, dtype=np.float64)
+    rad = np.hypot(k[:, None], k[None, :])
+    out = {"n": n, "budget_sweeps": budget, "target_rel_residual": target, "sweeps_from_previous": base,
+           "windows": {}}
+    for w in wavelengths:
+        keep = rad <= 2.0 * n / float(w)  # wavelength >= w: unresolved, left at p_prev's error
+        sw = sweeps_to_reach(d, p_prev, target, keep=keep)
+        out["windows"][int(w)] = {"sweeps": sw, "speedup_bound": (base / sw) if sw > 0 else None}
+    return out
+
+
 def compact_residual(d: np.ndarray, p: np.ndarray) -> float:
     """||d - L p|| / ||d|| with p's continuous ring (interior arrays)."""
     q = np.pad(p, 1, mode="edge")
@@ -354,14 +459,17 @@ def projection_race(R: int, net, seed: int, start_step: int, steps: int, tol: fl
 
 
 def equal_residual_race(R: int, net, seed: int = 7, start_step: int = 100, budget: int = 16384,
-                        warm_sweeps: int = 20000, K: int = 40, check: int = 1024):
+                        warm_sweeps: int = 20000, K: int = 40, check: int = 1024,
+                        windows=(RECEPTIVE_FIELD, 2 * RECEPTIVE_FIELD, 4 * RECEPTIVE_FIELD, 8 * RECEPTIVE_FIELD)):
     """The first projection of step start_step + 2 of a seeded 64-vortex flow,
     after one step whose projections were solved for warm_sweeps sweeps (so
     the previous pressure is a converged one), from identical states: the
     relative residual plain Jacobi reaches in `budget` sweeps from zero and
     from the previous pressure, and the sweeps and time the network's seed
     needs to reach the previous-pressure start's residual.  Every run checks
-    the residual every `check` sweeps (a host synchronisation each)."""
+    the residual every `check` sweeps (a host synchronisation each).  Also the
+    bound of local_seed_bound on the same projection for each window side in
+    `windows`."""
     st = W.vortex_state(R, seed=seed)
 
     def state():
@@ -397,4 +505,14 @@ def equal_residual_race(R: int, net, seed: int = 7, start_step: int = 100, budge
         sw, rel, fb, sec = seeded_projection(s, None, 3, 1e-4, 0, check, fallback=False)
     out["multigrid_to_1e-4"] = {"sweeps": sw, "rel_residual": rel, "ms": sec * 1e3, "reached": rel <= 1e-4}
     out["target_rel_residual"] = target
+    # the same projection in the DCT model: its residuals from zero and from the
+    # previous pressure after the budget (to set beside the measured ones), and
+    # what an ideal seed resolving every wavelength below a window side gains
+    with state() as s:
+        _, u, v = s.download()
+        d = divergence(u, v)
+        prev = s.pressure()[1:-1, 1:-1].astype(np.float64)
+    bound = local_seed_bound(d, prev, budget, wavelengths=windows)
+    bound["model_from_zero"] = float(jacobi_residual_model(d, np.zeros_like(d), [budget])[0])
+    out["local_seed_bound"] = bound
     return out
