@@ -81,6 +81,7 @@ def bytes_per_cell_step(kd: int, kp: int, a_dens: bool, a_visc: bool) -> int:
 
 
 LAUNCH_MODE = 0  # fs2d_amd.LAUNCH_GRAPH; --launch eager sets 1
+GS_FIXED = {}  # Gauss-Seidel launch parameters fixed by --gs-fixed (empty: tuned)
 
 
 def fs2d_amd_copy_bandwidth() -> float:
@@ -95,7 +96,8 @@ def run_gpu(order: int, n: int, kd: int, kp: int, steps: int, warmup: int, state
 
     R = n + 2
     diff, visc, dt = params
-    s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order, launch_mode=LAUNCH_MODE)
+    s = fs2d_amd.Solver(R, diffuse_iterations=kd, pressure_iterations=kp, order=order, launch_mode=LAUNCH_MODE,
+                        **(GS_FIXED if order == 0 else {}))
     s.upload(*state)
     s.set_sources(None, None, None)
     s.step_n(warmup, diff, visc, dt)
@@ -338,10 +340,16 @@ def main() -> int:
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--launch", choices=("graph", "eager"), default="graph",
                     help="replay captured step graphs (default) or issue every launch")
+    ap.add_argument("--gs-fixed", default="",
+                    help="COLS,SWEEPS,LAG: fix the Gauss-Seidel launch policy instead of tuning it (for counter "
+                         "runs, whose serialised dispatches distort the tuner's timings)")
     ap.add_argument("--cpu-sample-n", type=int, default=2048, help="interior size of the timed CPU sample")
     args = ap.parse_args()
-    global LAUNCH_MODE
+    global LAUNCH_MODE, GS_FIXED
     LAUNCH_MODE = 1 if args.launch == "eager" else 0
+    if args.gs_fixed:
+        tw, m, lag = (int(x) for x in args.gs_fixed.split(","))
+        GS_FIXED = {"gs_tile_cols": tw, "gs_block": m, "gs_lag": lag}
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported (a single grid would "
               "shard as row strips with a halo exchange per stage, which is not built)", file=sys.stderr)
@@ -394,7 +402,8 @@ def main() -> int:
         "data": "synthetic (seeded 64-vortex velocity, smoothed-noise density)",
         "config": {"workload": f"stable-fluids step {n}x{n}+ring, dt=0.05, visc=1e-6, diff=1e-5, "
                                f"K_d=K_p={K}, {'Gauss-Seidel (reference order)' if order == 0 else 'Jacobi'}",
-                   "order": args.order, "n": n, "iterations": K, "launch": args.launch},
+                   "order": args.order, "n": n, "iterations": K, "launch": args.launch,
+                   **({"gs_fixed": args.gs_fixed} if args.gs_fixed else {})},
         "cell_updates_per_s": round(cells / sec_step, 1),
         "step_hbm_gbs_algorithmic": round(step_gbs, 1),
         "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
