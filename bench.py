@@ -232,12 +232,29 @@ def run_learned_guess():
                                  "time_speedup_bound_with_forward": round(
                                      plain["ms"] / (x["sweeps"] * per_sweep + fwd), 3)}
                         for w, x in b["windows"].items()}}
+        mr = r["multigrid_local_refinement"]
+        per_cycle = mr["ms_two_cycles"] - mr["ms_one_cycle"]
+        refine = {
+            "question": "a network refining one V-cycle's pressure: the V-cycles an ideal correction of every "
+                        "wavelength below W could replace, against its forward time",
+            "one_cycle_residual": {"measured": round(mr["measured_one_cycle"], 6),
+                                   "model": round(mr["model_one_cycle"], 6)},
+            "two_cycle_residual": round(mr["measured_two_cycles"], 7),
+            "cycle_factor": round(mr["cycle_factor"], 3) if mr["cycle_factor"] else None,
+            "ms_per_extra_cycle": round(per_cycle, 3), "network_forward_ms": round(fwd, 3),
+            "windows": {str(w): {"residual_after_ideal_correction": round(x["residual_after_ideal_correction"], 6),
+                                 "v_cycles_replaced_at_most": (round(x["v_cycles_replaced_at_most"], 3)
+                                                               if x["v_cycles_replaced_at_most"] is not None else None),
+                                 "ms_saved_at_most": (round(x["v_cycles_replaced_at_most"] * per_cycle, 3)
+                                                      if x["v_cycles_replaced_at_most"] is not None else None)}
+                        for w, x in mr["windows"].items()}}
         mg, mg4 = r["multigrid"], r["multigrid_to_1e-4"]
         out[f"n{R - 2}"]["multigrid_seed"] = {
             "sweeps": mg["sweeps"], "ms": round(mg["ms"], 3), "reached": mg["reached"],
             "rel_residual": round(mg["rel_residual"], 7), "speedup": round(plain["ms"] / max(mg["ms"], 1e-9), 2),
             "v_cycles_toward_1e-4": {"rel_residual": round(mg4["rel_residual"], 7), "ms": round(mg4["ms"], 3),
-                                     "reached": mg4["reached"]}}
+                                     "reached": mg4["reached"]},
+            "local_refinement_bound": refine}
     ms = net.time_forward(1, 1024, 1024, reps=10)
     fl = net.forward_flops(1, 1024, 1024)
     # useful work: the first layer reads 2 real input channels of its 16 and the

@@ -469,7 +469,8 @@ def equal_residual_race(R: int, net, seed: int = 7, start_step: int = 100, budge
     needs to reach the previous-pressure start's residual.  Every run checks
     the residual every `check` sweeps (a host synchronisation each).  Also the
     bound of local_seed_bound on the same projection for each window side in
-    `windows`."""
+    `windows`, and the same ideal local correction applied to the pressure
+    of one multigrid V-cycle instead (how many V-cycles it could replace)."""
     st = W.vortex_state(R, seed=seed)
 
     def state():
@@ -515,4 +516,27 @@ def equal_residual_race(R: int, net, seed: int = 7, start_step: int = 100, budge
     bound = local_seed_bound(d, prev, budget, wavelengths=windows)
     bound["model_from_zero"] = float(jacobi_residual_model(d, np.zeros_like(d), [budget])[0])
     out["local_seed_bound"] = bound
+    # a local network refining the multigrid seed: one V-cycle's pressure, the
+    # residual it leaves, what an ideal correction of every wavelength below W
+    # would leave, and the measured factor of a second V-cycle -- so the
+    # V-cycles such a correction could replace at most
+    with state() as s:
+        _, u, v = s.download()
+        d = divergence(u, v)
+        _, rel1, _, sec1 = seeded_projection(s, None, 3, target, 0, check, fallback=False)
+        p_mg = s.pressure()[1:-1, 1:-1].astype(np.float64)
+    with state() as s:
+        _, rel2, _, sec2 = seeded_projection(s, None, 3, 0.5 * rel1, 0, check, fallback=False)
+    n = d.shape[0]
+    k = np.arange(n, dtype=np.float64)
+    rad = np.hypot(k[:, None], k[None, :])
+    r_mg = float(jacobi_residual_model(d, p_mg, [0])[0])
+    refine = {"measured_one_cycle": rel1, "model_one_cycle": r_mg, "measured_two_cycles": rel2,
+              "cycle_factor": rel1 / rel2 if rel2 > 0 else None, "ms_one_cycle": sec1 * 1e3,
+              "ms_two_cycles": sec2 * 1e3, "windows": {}}
+    for w in windows:
+        r_w = float(jacobi_residual_model(d, p_mg, [0], keep=rad <= 2.0 * n / float(w))[0])
+        saved = (np.log(r_mg / r_w) / np.log(rel1 / rel2)) if (rel2 > 0 and rel1 > rel2 and r_w > 0) else None
+        refine["windows"][int(w)] = {"residual_after_ideal_correction": r_w, "v_cycles_replaced_at_most": saved}
+    out["multigrid_local_refinement"] = refine
     return out
