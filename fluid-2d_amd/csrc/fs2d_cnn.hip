@@ -224,9 +224,9 @@ int acts_for(fs2d_net* n, int N, int H, int W, Acts** out) {
     const B list[] = {
         {"x0", p1 * 16, 2},  {"a1", p1 * 16, 2},  {"e1", p1 * 16, 2},  {"p1", p2 * 16, 2},  {"a2", p2 * 32, 2},
         {"e2", p2 * 32, 2},  {"p2", p3 * 32, 2},  {"a3", p3 * 64, 2},  {"e3", p3 * 64, 2},  {"p3", p4 * 64, 2},
-        {"a4", p4 * 128, 2}, {"e4", p4 * 128, 2}, {"u3", p3 * 128, 2}, {"c3", p3 * 192, 2}, {"b3", p3 * 64, 2},
-        {"d3", p3 * 64, 2},  {"u2", p2 * 64, 2},  {"c2", p2 * 96, 2},  {"b2", p2 * 32, 2},  {"d2", p2 * 32, 2},
-        {"u1", p1 * 32, 2},  {"c1", p1 * 48, 2},  {"b1", p1 * 16, 2},  {"d1", p1 * 16, 2},  {"o", p1 * 16, 4},
+        {"a4", p4 * 128, 2}, {"e4", p4 * 128, 2}, {"c3", p3 * 192, 2}, {"b3", p3 * 64, 2},
+        {"d3", p3 * 64, 2},  {"c2", p2 * 96, 2},  {"b2", p2 * 32, 2},  {"d2", p2 * 32, 2},
+        {"c1", p1 * 48, 2},  {"b1", p1 * 16, 2},  {"d1", p1 * 16, 2},  {"o", p1 * 16, 4},
         {"prev", p1, 4},     {"tgt", p1, 4},      {"scale", N, 4},
         // gradients
         {"dO", p1 * 16, 2},  {"gA", p1 * 48, 2},  {"gB", p1 * 48, 2},  {"gS1", p1 * 16, 2}, {"gS2", p2 * 32, 2},
@@ -251,17 +251,14 @@ int acts_for(fs2d_net* n, int N, int H, int W, Acts** out) {
 int forward(fs2d_net* n, Acts& a) {
     const int N = a.N, H = a.H, W = a.W;
     int rc;
-    const long p1 = static_cast<long>(N) * H * W, p2 = p1 / 4, p3 = p1 / 16;
     auto pool = [&](const bf16* x, bf16* y, int h, int w, int C) {
         hipLaunchKernelGGL(k_avgpool2, dim3(blocks_for(static_cast<long>(N) * h * w * C / 4)), dim3(256), 0, n->stream,
                            x, y, N, h, w, C);
     };
-    auto up = [&](const bf16* x, bf16* y, int h, int w, int C) {
-        hipLaunchKernelGGL(k_upsample2, dim3(blocks_for(static_cast<long>(N) * h * w * C * 4)), dim3(256), 0,
-                           n->stream, x, y, N, h, w, C);
-    };
-    auto cat = [&](const bf16* x, const bf16* y, bf16* out, long np, int Ca, int Cb) {
-        hipLaunchKernelGGL(k_concat, dim3(blocks_for(np * (Ca + Cb))), dim3(256), 0, n->stream, x, y, out, np, Ca, Cb);
+    // upsample the coarse level and concatenate the skip in one pass
+    auto upcat = [&](const bf16* coarse, const bf16* skip, bf16* out, int h, int w, int Ca, int Cb) {
+        hipLaunchKernelGGL(k_upcat, dim3(blocks_for(static_cast<long>(N) * h * w * 4 * ((Ca + Cb) / 8))), dim3(256), 0,
+                           n->stream, coarse, skip, out, N, h, w, Ca, Cb);
     };
     if ((rc = conv(n, E1A, a("x0"), a("a1"), N, H, W))) return rc;
     if ((rc = conv(n, E1B, a("a1"), a("e1"), N, H, W))) return rc;
@@ -274,16 +271,13 @@ int forward(fs2d_net* n, Acts& a) {
     pool(a("e3"), a("p3"), H / 4, W / 4, 64);
     if ((rc = conv(n, E4A, a("p3"), a("a4"), N, H / 8, W / 8))) return rc;
     if ((rc = conv(n, E4B, a("a4"), a("e4"), N, H / 8, W / 8))) return rc;
-    up(a("e4"), a("u3"), H / 8, W / 8, 128);
-    cat(a("u3"), a("e3"), a("c3"), p3, 128, 64);
+    upcat(a("e4"), a("e3"), a("c3"), H / 8, W / 8, 128, 64);
     if ((rc = conv(n, D3A, a("c3"), a("b3"), N, H / 4, W / 4))) return rc;
     if ((rc = conv(n, D3B, a("b3"), a("d3"), N, H / 4, W / 4))) return rc;
-    up(a("d3"), a("u2"), H / 4, W / 4, 64);
-    cat(a("u2"), a("e2"), a("c2"), p2, 64, 32);
+    upcat(a("d3"), a("e2"), a("c2"), H / 4, W / 4, 64, 32);
     if ((rc = conv(n, D2A, a("c2"), a("b2"), N, H / 2, W / 2))) return rc;
     if ((rc = conv(n, D2B, a("b2"), a("d2"), N, H / 2, W / 2))) return rc;
-    up(a("d2"), a("u1"), H / 2, W / 2, 32);
-    cat(a("u1"), a("e1"), a("c1"), p1, 32, 16);
+    upcat(a("d2"), a("e1"), a("c1"), H / 2, W / 2, 32, 16);
     if ((rc = conv(n, D1A, a("c1"), a("b1"), N, H, W))) return rc;
     if ((rc = conv(n, D1B, a("b1"), a("d1"), N, H, W))) return rc;
     if ((rc = conv(n, OUT, a("d1"), a.f("o"), N, H, W, true))) return rc;

@@ -306,25 +306,45 @@ __device__ inline void up_taps(int Y, int h, int& a, int& b, float& wa, float& w
     }
 }
 
-__global__ __launch_bounds__(256) void k_upsample2(const bf16* __restrict__ x, bf16* __restrict__ y, int N, int h, int w,
-                                                  int C) {
-    const int H = 2 * h, W = 2 * w;
-    const long n = static_cast<long>(N) * H * W * C;
+// The decoder's input in one pass: y[p] = (bilinear x2 of x at p -- up_taps'
+// two rows and two columns, wy_a (wx_a A + wx_b B) + wy_b (wx_a C + wx_b D) in
+// FP32, rounded to bfloat16 -- channels [0, Ca); skip[p], channels [Ca, Ca + Cb)).
+// One thread per (pixel, 8-channel group) with 16-byte loads and stores; Ca and
+// Cb are multiples of 8.
+typedef short short8_t __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(256) void k_upcat(const bf16* __restrict__ x, const bf16* __restrict__ skip,
+                                              bf16* __restrict__ y, int N, int h, int w, int Ca, int Cb) {
+    const int H = 2 * h, W = 2 * w, C = Ca + Cb, G = C / 8;
+    const long n = static_cast<long>(N) * H * W * G;
     for (long t = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; t < n; t += static_cast<long>(gridDim.x) * 256) {
-        const int c = static_cast<int>(t % C);
-        long r = t / C;
-        const int X = static_cast<int>(r % W);
-        r /= W;
-        const int Y = static_cast<int>(r % H);
-        const long b = r / H;
-        int ya, yb, xa, xb;
-        float wya, wyb, wxa, wxb;
-        up_taps(Y, h, ya, yb, wya, wyb);
-        up_taps(X, w, xa, xb, wxa, wxb);
-        const bf16* q = x + b * h * w * C + c;
-        const float v = wya * (wxa * bf2f(q[(static_cast<long>(ya) * w + xa) * C]) + wxb * bf2f(q[(static_cast<long>(ya) * w + xb) * C])) +
-                        wyb * (wxa * bf2f(q[(static_cast<long>(yb) * w + xa) * C]) + wxb * bf2f(q[(static_cast<long>(yb) * w + xb) * C]));
-        y[t] = f2bf(v);
+        const int g = static_cast<int>(t % G);
+        const long p = t / G;
+        short8_t out;
+        if (8 * g < Ca) {
+            const int X = static_cast<int>(p % W);
+            const long r = p / W;
+            const int Y = static_cast<int>(r % H);
+            const long b = r / H;
+            int ya, yb, xa, xb;
+            float wya, wyb, wxa, wxb;
+            up_taps(Y, h, ya, yb, wya, wyb);
+            up_taps(X, w, xa, xb, wxa, wxb);
+            const bf16* q = x + b * h * w * Ca + 8 * g;
+            const short8_t A = *reinterpret_cast<const short8_t*>(q + (static_cast<long>(ya) * w + xa) * Ca);
+            const short8_t B = *reinterpret_cast<const short8_t*>(q + (static_cast<long>(ya) * w + xb) * Ca);
+            const short8_t Cc = *reinterpret_cast<const short8_t*>(q + (static_cast<long>(yb) * w + xa) * Ca);
+            const short8_t D = *reinterpret_cast<const short8_t*>(q + (static_cast<long>(yb) * w + xb) * Ca);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = wya * (wxa * bf2f(static_cast<bf16>(A[e])) + wxb * bf2f(static_cast<bf16>(B[e]))) +
+                                wyb * (wxa * bf2f(static_cast<bf16>(Cc[e])) + wxb * bf2f(static_cast<bf16>(D[e])));
+                out[e] = static_cast<short>(f2bf(v));
+            }
+        } else {
+            out = *reinterpret_cast<const short8_t*>(skip + p * Cb + 8 * g - Ca);
+        }
+        *reinterpret_cast<short8_t*>(y + p * C + 8 * g) = out;
     }
 }
 
@@ -358,18 +378,6 @@ __global__ __launch_bounds__(256) void k_upsample2_bwd(const bf16* __restrict__ 
             }
         }
         dx[t] = f2bf(s);
-    }
-}
-
-// y[p] = (a[p][0..Ca), b[p][0..Cb)).
-__global__ __launch_bounds__(256) void k_concat(const bf16* __restrict__ a, const bf16* __restrict__ b,
-                                               bf16* __restrict__ y, long npix, int Ca, int Cb) {
-    const int C = Ca + Cb;
-    const long n = npix * C;
-    for (long t = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; t < n; t += static_cast<long>(gridDim.x) * 256) {
-        const long p = t / C;
-        const int c = static_cast<int>(t % C);
-        y[t] = c < Ca ? a[p * Ca + c] : b[p * Cb + c - Ca];
     }
 }
 
