@@ -540,3 +540,18 @@ def equal_residual_race(R: int, net, seed: int = 7, start_step: int = 100, budge
         refine["windows"][int(w)] = {"residual_after_ideal_correction": r_w, "v_cycles_replaced_at_most": saved}
     out["multigrid_local_refinement"] = refine
     return out
+
+
+if __name__ == "__main__":
+    # python -m fs2d_amd.pressure_net [side ...]: the trained network's forward
+    # time (HIP events, 20 passes) and FLOPs at each square side (default 1024)
+    import json
+    import sys
+
+    net = PressureNet(seed=0)
+    net.load(DEFAULT_PARAMS)
+    for side in (int(a) for a in (sys.argv[1:] or ["1024"])):
+        ms = net.time_forward(1, side, side, reps=20)
+        print(json.dumps({"side": side, "forward_ms": round(ms, 4), "flops": net.forward_flops(1, side, side)}),
+              flush=True)
+    net.close()
