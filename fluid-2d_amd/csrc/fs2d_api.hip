@@ -523,31 +523,6 @@ int joint_levels(const JointPlans& j) {
     return static_cast<int>(std::unique(lv.begin(), lv.end()) - lv.begin());
 }
 
-// A projection whose pressure solve (Gauss-Seidel units) computes the
-// divergence of (u, v) where each unit loads its right-hand side, so no
-// divergence pass runs and none is stored; otherwise project().
-int project_onload(fs2d_handle* h, int K) {
-    if (K > 0 && h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h)) {
-        const GsPlan* plan = nullptr;
-        FS2D_HIP(h, gs_plan_for(h, K, SWEEP_PRESSURE, &plan));
-        if (plan->M > 0) {
-            const Scalars s = scalars(h->R, 1.0f);
-            FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
-            SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 0);
-            A.du = h->u();
-            A.dv = h->v();
-            A.neg_half_h = s.neg_half_h;
-            FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
-            h->pressure_zero = false;
-            FS2D_HIP(h, launch_gradient(h->stream, h->u(), h->v(), h->f[F_P], h->R, h->P, s.h, 0, h->max_bits,
-                                        h->speed_slots));
-            h->speed_valid = true;
-            return FS2D_OK;
-        }
-    }
-    return project(h, K);
-}
-
 // The three diffuses of a step relaxed together (Gauss-Seidel unit plans):
 // density in place, u and v on fresh iterates so that the density scatter can
 // still read the pre-step velocity (src/fluid_solver_cpu.cpp:16-24).  Same
@@ -588,7 +563,10 @@ int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
                                         h->max_bits, h->speed_slots));
             h->speed_valid = true;
             if ((rc = advect_velocity(h, dt))) return rc;
-            return project_onload(h, Kp);
+            // the second projection from a stored divergence: its units then
+            // read one right-hand side field instead of two (faster than the
+            // on-load divergence, which only the pipelined first solve needs)
+            return project(h, Kp);
         }
         const GsPlan* plan = nullptr;
         FS2D_HIP(h, gs_plan_for(h, Kd, SWEEP_DIFFUSE, &plan));
@@ -1083,22 +1061,10 @@ static int fs2d_hip_time_pressure_sweeps_impl(fs2d_handle* h, int reps, float* m
     if (!h || reps <= 0 || !ms) return fail(h, FS2D_ERR_ARGUMENT, "time_pressure_sweeps: bad argument");
     const Scalars s = scalars(h->R, 1.0f);
     const int K = h->prm.pressure_iterations;
-    const GsPlan* plan = nullptr;
-    if (K > 0 && h->prm.sweep_order == FS2D_ORDER_GS_LEX && gs_units(h))
-        FS2D_HIP(h, gs_plan_for(h, K, SWEEP_PRESSURE, &plan));
     int rc;
-    auto once = [&]() -> int {
-        if (plan && plan->M > 0) {  // as a step's second projection: divergence computed on load
-            FS2D_HIP(h, launch_fill(h->stream, h->f[F_P], 0u, h->field_bytes / 4));
-            SweepArgs A = sweep_args(h->f[F_P], h->f[F_D], h->R, h->P, SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 0);
-            A.du = h->u();
-            A.dv = h->v();
-            A.neg_half_h = s.neg_half_h;
-            FS2D_HIP(h, launch_gs_solve(h->stream, *plan, A));
-            return FS2D_OK;
-        }
-        return linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1);
-    };
+    // as a step's second projection runs it: zero fill, then the sweeps from
+    // the stored divergence
+    auto once = [&]() -> int { return linear_solve(h, h->f[F_P], h->f[F_D], SWEEP_PRESSURE, 0.f, 0.f, BND_CONT, K, 1); };
     FS2D_HIP(h, launch_divergence(h->stream, h->f[F_D], h->u(), h->v(), h->R, h->P, s.neg_half_h, nullptr));
     if ((rc = once())) return rc;  // warm
     FS2D_HIP(h, hipEventRecord(h->ev0, h->stream));
@@ -1309,7 +1275,7 @@ static int fs2d_hip_launches_per_step_impl(fs2d_handle* h, int* launches) {
     if (joint_plans(h, Kd, Kp, &joint) != hipSuccess) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
     if (joint.d) {
         // pipelined: add x3, fill p, the joint level launches, scatter, gradient +
-        // fold, gather, then fill p, the second solve's levels, gradient + fold
+        // fold, gather, then the divergence (filling p), the second solve's levels, gradient + fold
         *launches = 3 + 1 + joint_levels(joint) + scatter_launches(h->R) + 2 + 1 + 1 + p + 2;
         return FS2D_OK;
     }

@@ -93,9 +93,9 @@ int fs2d_hip_project_seeded(struct fs2d_handle* h, fs2d_net* net, int seed, floa
                             int fallback, int* sweeps_run, double* rel_residual, int* fell_back);
 /* Relative compact Poisson residual ||d - L p||_2 / ||d||_2 of the handle's
  * stored divergence and pressure fields (FP64 sums): those of the last
- * fs2d_hip_stage_project or seeded projection.  (A Gauss-Seidel step's
- * projections compute the divergence where the pressure solve loads it and
- * store none.) */
+ * fs2d_hip_stage_project, seeded projection or step (whose last projection
+ * stores its divergence; a Gauss-Seidel step's first projection computes it
+ * where the pressure solve loads it and stores none). */
 int fs2d_hip_pressure_residual(struct fs2d_handle* h, double* rel_residual);
 
 #ifdef __cplusplus

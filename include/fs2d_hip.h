@@ -146,8 +146,8 @@ int fs2d_hip_download_pressure(fs2d_handle* h, float* p);
 /* The handle's HIP stream (a hipStream_t), for callers that record events. */
 void* fs2d_hip_stream(fs2d_handle* h);
 /* Enqueue only the relaxation sweeps of one pressure solve (K_p sweeps on the
- * current velocity's divergence; with Gauss-Seidel units as a step's second
- * projection runs it, the divergence computed on load, the zero fill included)
+ * current velocity's divergence, stored first; as a step's second projection
+ * runs it, the zero fill included)
  * `reps` times and return the mean time of one solve in milliseconds,
  * measured with HIP events on the handle's stream.  The state's velocity is
  * not modified. */

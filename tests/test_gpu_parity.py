@@ -358,7 +358,8 @@ def test_launch_policy_and_launch_count(gsb, tw, lag):
     lv = {u[0][0] + u[0][1] + u[0][2] for u in gs_unit_plan(R, K, tw, m)}
     joint = len(lv | {x + off for x in lv})
     assert pol[2]["levels"] == joint, (pol[2], joint)
-    # add x3, fill p, joint levels, scatter, gradient + fold, gather, fill p, p levels, gradient + fold
+    # add x3, fill p, joint levels, scatter, gradient + fold, gather, divergence (filling p), p levels,
+    # gradient + fold
     assert launches == 3 + 1 + joint + scatter_launches(R) + 2 + 1 + 1 + p + 2, (d, p, joint, pol)
 
 
