@@ -531,9 +531,8 @@ int step_shared_diffuse(fs2d_handle* h, float diff, float visc, float dt) {
     const int Kd = h->prm.diffuse_iterations, Kp = h->prm.pressure_iterations;
     const int R = h->R, P = h->P;
     int rc;
-    FS2D_HIP(h, launch_add_sources(h->stream, h->rho(), h->f[F_X0], h->f[F_RS], R, P, dt));
-    FS2D_HIP(h, launch_add_sources_split(h->stream, h->f[F_UN], h->f[F_X0U], h->u(), h->f[F_US], R, P, dt));
-    FS2D_HIP(h, launch_add_sources_split(h->stream, h->f[F_VN], h->f[F_X0V], h->v(), h->f[F_VS], R, P, dt));
+    FS2D_HIP(h, launch_add_sources3(h->stream, h->rho(), h->f[F_X0], h->f[F_RS], h->f[F_UN], h->f[F_X0U], h->u(),
+                                    h->f[F_US], h->f[F_VN], h->f[F_X0V], h->v(), h->f[F_VS], R, P, dt));
     if (Kd > 0) {
         const float ad = dt * cells(R) * diff, av = dt * cells(R) * visc;
         const float dd = 1.f + 4.f * ad, dv = 1.f + 4.f * av;
@@ -1274,13 +1273,15 @@ static int fs2d_hip_launches_per_step_impl(fs2d_handle* h, int* launches) {
     JointPlans joint;
     if (joint_plans(h, Kd, Kp, &joint) != hipSuccess) return fail(h, FS2D_ERR_HIP, "launches_per_step: plan failed");
     if (joint.d) {
-        // pipelined: add x3, fill p, the joint level launches, scatter, gradient +
-        // fold, gather, then the divergence (filling p), the second solve's levels, gradient + fold
-        *launches = 3 + 1 + joint_levels(joint) + scatter_launches(h->R) + 2 + 1 + 1 + p + 2;
+        // pipelined: the three adds (one launch), fill p, the joint level launches,
+        // scatter, gradient + fold, gather, then the divergence (filling p), the
+        // second solve's levels, gradient + fold
+        *launches = 1 + 1 + joint_levels(joint) + scatter_launches(h->R) + 2 + 1 + 1 + p + 2;
         return FS2D_OK;
     }
-    // add x3, diffuse (x3, or x1 shared), scatter, project x2 (div + solve + grad), gather
-    *launches = 3 + (shared ? d : 3 * d) + scatter_launches(h->R) + 2 * (2 + p) + 1;
+    // adds (one launch shared, x3 otherwise), diffuse (x3, or x1 shared), scatter,
+    // project x2 (div + solve + grad), gather
+    *launches = (shared ? 1 : 3) + (shared ? d : 3 * d) + scatter_launches(h->R) + 2 * (2 + p) + 1;
     return FS2D_OK;
 }
 

@@ -37,6 +37,54 @@ __global__ __launch_bounds__(BX* BY) void k_add_sources_split(float* __restrict_
     if (interior(R, i, j)) cells::add_sources_split(out, x0, x, s, R, P, dt, i, j);
 }
 
+// The step's first stage in one pass (src/fluid_solver_cpu.cpp:16, :21-22 with
+// the copies diffuse takes, :102): density in place, rho += dt * rs, with its
+// copy; u and v into fresh iterates, un = u + dt * us, with their copies, and
+// the fresh iterates' ring copied from u and v (the stale ring the first sweep
+// reads).  A thread takes 4 columns of a row, with 16-byte accesses where all
+// 4 are interior (rows start 256-byte aligned, so quads are).
+__global__ __launch_bounds__(256) void k_add_sources3(float* __restrict__ rho, float* __restrict__ x0,
+                                                     const float* __restrict__ rs, float* __restrict__ un,
+                                                     float* __restrict__ x0u, const float* __restrict__ u,
+                                                     const float* __restrict__ us, float* __restrict__ vn,
+                                                     float* __restrict__ x0v, const float* __restrict__ v,
+                                                     const float* __restrict__ vs, int R, int P, float dt) {
+    const int j0 = 4 * (blockIdx.x * 256 + threadIdx.x), i = blockIdx.y;
+    if (j0 >= R) return;
+    const size_t o = (size_t)i * P + j0;
+    const int n = R - 2;
+    if (i >= 1 && i <= n && j0 >= 1 && j0 + 3 <= n) {
+        const float4 a = *reinterpret_cast<const float4*>(rho + o), sa = *reinterpret_cast<const float4*>(rs + o);
+        const float4 b = *reinterpret_cast<const float4*>(u + o), sb = *reinterpret_cast<const float4*>(us + o);
+        const float4 c = *reinterpret_cast<const float4*>(v + o), sc = *reinterpret_cast<const float4*>(vs + o);
+        const float4 ra = make_float4(a.x + dt * sa.x, a.y + dt * sa.y, a.z + dt * sa.z, a.w + dt * sa.w);
+        const float4 rb = make_float4(b.x + dt * sb.x, b.y + dt * sb.y, b.z + dt * sb.z, b.w + dt * sb.w);
+        const float4 rc = make_float4(c.x + dt * sc.x, c.y + dt * sc.y, c.z + dt * sc.z, c.w + dt * sc.w);
+        *reinterpret_cast<float4*>(rho + o) = ra;
+        *reinterpret_cast<float4*>(x0 + o) = ra;
+        *reinterpret_cast<float4*>(un + o) = rb;
+        *reinterpret_cast<float4*>(x0u + o) = rb;
+        *reinterpret_cast<float4*>(vn + o) = rc;
+        *reinterpret_cast<float4*>(x0v + o) = rc;
+        return;
+    }
+    for (int j = j0; j < min(j0 + 4, R); ++j) {
+        const size_t q = (size_t)i * P + j;
+        if (i >= 1 && i <= n && j >= 1 && j <= n) {
+            const float ra = rho[q] + dt * rs[q], rb = u[q] + dt * us[q], rc = v[q] + dt * vs[q];
+            rho[q] = ra;
+            x0[q] = ra;
+            un[q] = rb;
+            x0u[q] = rb;
+            vn[q] = rc;
+            x0v[q] = rc;
+        } else {  // the ring, corners included
+            un[q] = u[q];
+            vn[q] = v[q];
+        }
+    }
+}
+
 // Largest finite |u|, |v| over the interior, as float bits (non-negative
 // floats order like their bit patterns).  Bounds how far a source moves in the
 // density scatter.
@@ -167,6 +215,14 @@ hipError_t launch_add_sources(hipStream_t s, float* x, float* x0, const float* s
 hipError_t launch_add_sources_split(hipStream_t s, float* out, float* x0, const float* x, const float* src, int R,
                                     int P, float dt) {
     hipLaunchKernelGGL(k_add_sources_split, grid2d(R), dim3(BX, BY), 0, s, out, x0, x, src, R, P, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_sources3(hipStream_t s, float* rho, float* x0, const float* rs, float* un, float* x0u,
+                               const float* u, const float* us, float* vn, float* x0v, const float* v,
+                               const float* vs, int R, int P, float dt) {
+    const dim3 grid((R + 4 * 256 - 1) / (4 * 256), R);
+    hipLaunchKernelGGL(k_add_sources3, grid, dim3(256), 0, s, rho, x0, rs, un, x0u, u, us, vn, x0v, v, vs, R, P, dt);
     return hipGetLastError();
 }
 

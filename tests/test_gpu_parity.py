@@ -358,9 +358,9 @@ def test_launch_policy_and_launch_count(gsb, tw, lag):
     lv = {u[0][0] + u[0][1] + u[0][2] for u in gs_unit_plan(R, K, tw, m)}
     joint = len(lv | {x + off for x in lv})
     assert pol[2]["levels"] == joint, (pol[2], joint)
-    # add x3, fill p, joint levels, scatter, gradient + fold, gather, divergence (filling p), p levels,
+    # adds (one launch), fill p, joint levels, scatter, gradient + fold, gather, divergence (filling p), p levels,
     # gradient + fold
-    assert launches == 3 + 1 + joint + scatter_launches(R) + 2 + 1 + 1 + p + 2, (d, p, joint, pol)
+    assert launches == 1 + 1 + joint + scatter_launches(R) + 2 + 1 + 1 + p + 2, (d, p, joint, pol)
 
 
 # ------------------------------------------------ the reference's own outputs
