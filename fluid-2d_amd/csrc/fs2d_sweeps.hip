@@ -391,6 +391,13 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
         // lane 0, which is active while s < TW; it enters as the DPP shift's
         // old value (lane 0 has no source lane).
         static_assert(GS_ROWS == 64 && TW % 64 == 0 && 64 % L == 0, "the step phases assume 64-row tiles");
+        // The west neighbour of a lane's first cell (step r) lies in the tile's
+        // west halo column.  Waves w' <= w never write that cell (they would
+        // reach it at a negative tile column); waves w' > w write it at tick
+        // r - 1 + (2 + L) w' - 2 w, after this wave's step r (tick r + L w).
+        // So its value at the unit's start is the one step r reads: read it once.
+        __syncthreads();
+        const float w_halo = X[me0 + r - 1];
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
@@ -399,14 +406,13 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
             const int x0 = me0 + s0, b0 = bme0 + s0;
             float so[L], e[L], bb[L];
             if (s0 < 64) {
-                float nh[L], wh[L];
+                float nh[L];
 #pragma unroll
                 for (int q = 0; q < L; ++q) {
                     so[q] = X[x0 + q + XP];
                     e[q] = X[x0 + q + 1];
                     bb[q] = Bs[b0 + q];
                     nh[q] = X[x0 + q - XP];
-                    wh[q] = X[x0 + q - 1];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -414,7 +420,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     const int s = s0 + q;
                     const float nv = __int_as_float(
                         __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    const float wv = (r == s) ? wh[q] : xprev;
+                    const float wv = (r == s) ? w_halo : xprev;
                     const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, dv);
                     if (r <= s) X[x0 + q] = v;
                     // xprev is consumed only where this lane was active one
