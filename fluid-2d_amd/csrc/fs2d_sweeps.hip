@@ -485,6 +485,11 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
         const bool w_virtual = kpos && c_left >= 0;
         const bool rows_edge = GS_ROWS * I + 1 - k_hi <= 1 || GS_ROWS * I + GS_ROWS - k_lo >= n;
         const bool cols_edge = TW * J + 1 - k_hi <= 1 || TW * J + TW - k_lo >= n;
+        // West of the lane's first cell (c_lo, at step c_lo + r): the tile's west
+        // halo, written only by later waves after that step, or the ring, which
+        // no sweep writes -- read once (as in the plain units).
+        __syncthreads();
+        const float w_first = X[me0 + c_lo + r - 1];
         auto sweeps = [&](auto rows_t, auto cols_t) {
             constexpr bool RE = decltype(rows_t)::value, CE = decltype(cols_t)::value;
             for (int it = 0; it < NINT; ++it) {
@@ -492,14 +497,13 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                 const int s0 = it * L - L * w;
                 if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
                 const int x0 = me0 + s0, b0 = bme0 + s0;
-                float so[L], e[L], bb[L], nh[L], wh[L], self[L];
+                float so[L], e[L], bb[L], nh[L], self[L];
 #pragma unroll
                 for (int q = 0; q < L; ++q) {
                     so[q] = X[x0 + q + XP];
                     e[q] = X[x0 + q + 1];
                     bb[q] = Bs[b0 + q];
                     nh[q] = X[x0 + q - XP];
-                    wh[q] = X[x0 + q - 1];
                     self[q] = X[x0 + q];
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -517,10 +521,10 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     }
                     if constexpr (CE) {
                         const float lrv = __uint_as_float(__float_as_uint(self[q]) ^ lr_sign);
-                        wv = (c == c_lo) ? (w_virtual ? lrv : wh[q]) : xprev;
+                        wv = (c == c_lo) ? (w_virtual ? lrv : w_first) : xprev;
                         ev = (kpos && c == c_right) ? lrv : e[q];
                     } else {
-                        wv = (c == c_lo) ? wh[q] : xprev;
+                        wv = (c == c_lo) ? w_first : xprev;
                     }
                     const float v = relax_k<KIND>(bb[q], nv, sv, wv, ev, a, dv);
                     if (c >= c_lo && c <= c_hi) X[x0 + q] = v;
