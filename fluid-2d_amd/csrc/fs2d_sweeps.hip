@@ -825,6 +825,15 @@ hipError_t gs_unit_plan_build(GsPlan* plan, int R, int K, int TW, int M, int lag
                 if (any) by_level[I + J + kap].push_back(make_int2((I << 16) | J, kap));
             }
     }
+    // Within a level, units that meet the ring first: they are the slower ones
+    // (k_gs_unit's edge loop), so in a level holding more units than the CUs run
+    // at once they go in the first round rather than stretch the last one.
+    auto meets_ring = [&](const int2& u) {
+        const int I = u.x >> 16, J = u.x & 0xffff, k_lo = u.y * M, k_hi = std::min(k_lo + M, K) - 1;
+        return !(GS_ROWS * I + 1 - k_hi >= 2 && GS_ROWS * I + GS_ROWS - k_lo <= n - 1 && TW * J + 1 - k_hi >= 2 &&
+                 TW * J + TW - k_lo <= n - 1);
+    };
+    for (auto& units : by_level) std::stable_partition(units.begin(), units.end(), meets_ring);
     std::vector<int> offsets, level_of;
     std::vector<int2> work;
     for (size_t lv = 0; lv < by_level.size(); ++lv) {
