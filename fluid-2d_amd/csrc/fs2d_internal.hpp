@@ -89,12 +89,12 @@ struct SweepSet {
 hipError_t launch_add_sources(hipStream_t s, float* x, float* x0, const float* src, int R, int P, float dt);
 // out = x + dt * src and x0 = out on the interior, out's ring = x's ring; x is
 // left untouched (the density scatter still needs the pre-step velocity).
+hipError_t launch_add_sources_split(hipStream_t s, float* out, float* x0, const float* x, const float* src, int R,
+                                    int P, float dt);
 // The density's add_sources and u's and v's add_sources_split in one launch.
 hipError_t launch_add_sources3(hipStream_t s, float* rho, float* x0, const float* rs, float* un, float* x0u,
                                const float* u, const float* us, float* vn, float* x0v, const float* v,
                                const float* vs, int R, int P, float dt);
-hipError_t launch_add_sources_split(hipStream_t s, float* out, float* x0, const float* x, const float* src, int R,
-                                    int P, float dt);
 hipError_t launch_max_speed(hipStream_t s, unsigned* out_bits, const float* u, const float* v, int R, int P);
 // Density advection (fs2d_scatter.hip) into out (interior and ring); max_bits
 // holds max |u|, |v|.  sort_scratch (scatter_sort_bytes(R) bytes, may be null
