@@ -1224,6 +1224,17 @@ int fs2d_gs_pipeline_offset(int rows, int diffuse_iterations, int pressure_itera
     return FS2D_OK;
 }
 
+int fs2d_division_plan(float denom, float* inv, float* inv_lo, float* lo, int* checked) {
+    if (!inv || !inv_lo || !lo || !checked) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_division_plan: null output");
+    SweepArgs A = sweep_args(nullptr, nullptr, 3, 64, SWEEP_DIFFUSE, 0.f, denom, BND_CONT, 1, 0);
+    two_product_quotient(A);
+    *inv = A.inv_denom;
+    *inv_lo = A.inv_denom_lo;
+    *lo = A.div_lo;
+    *checked = A.div_checked;
+    return FS2D_OK;
+}
+
 static int fs2d_hip_copy_bandwidth_impl(long long bytes, int reps, float* gbs) {
     if (bytes < 1024 || reps < 1 || !gbs) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_hip_copy_bandwidth: bad argument");
     const size_t n4 = static_cast<size_t>(bytes) / 16;

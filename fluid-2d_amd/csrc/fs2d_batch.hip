@@ -60,8 +60,8 @@ struct BatchSim {
     int launches;     // K / M Jacobi launches per solve (diffuse and pressure)
     int cell_blocks;  // 64 x 4 blocks over the interior
     float dt, dt0, h, neg_half_h;
-    float a_rho, denom_rho, inv_rho, lo_rho;
-    float a_vel, denom_vel, inv_vel, lo_vel;
+    float a_rho, denom_rho, inv_rho, lo_rho, inv_lo_rho;
+    float a_vel, denom_vel, inv_vel, lo_vel, inv_lo_vel;
 };
 
 constexpr int CBX = 64, CBY = 4;      // single-pass stage blocks
@@ -224,6 +224,7 @@ __device__ __forceinline__ void kb_jacobi_entry(const BatchSim* __restrict__ sim
         A.a = rho ? S.a_rho : S.a_vel;
         A.denom = rho ? S.denom_rho : S.denom_vel;
         A.inv_denom = rho ? S.inv_rho : S.inv_vel;
+        A.inv_denom_lo = rho ? S.inv_lo_rho : S.inv_lo_vel;
         A.div_lo = rho ? S.lo_rho : S.lo_vel;
         A.ring = ring_of(f == 0 ? BND_CONT : (f == 1 ? BND_OPP_H : BND_OPP_V));
     }
@@ -751,8 +752,10 @@ static int fs2d_hip_batch_create_impl(int count, const fs2d_batch_sim* sims, con
         const SweepArgs dv = sweep_args(nullptr, nullptr, S.R, S.P, SWEEP_DIFFUSE, S.a_vel, S.denom_vel, BND_CONT, 2, 0);
         S.inv_rho = dr.inv_denom;
         S.lo_rho = dr.div_lo;
+        S.inv_lo_rho = dr.inv_denom_lo;
         S.inv_vel = dv.inv_denom;
         S.lo_vel = dv.div_lo;
+        S.inv_lo_vel = dv.inv_denom_lo;
         const int n = S.R - 2;
         const int gx = (n + CBX - 1) / CBX, gy = (n + CBY - 1) / CBY;
         S.cell_blocks = gx * gy;

@@ -54,6 +54,8 @@ struct SweepArgs {
     const float* du;
     const float* dv;
     float neg_half_h;
+    float inv_denom_lo;  // RN(1 / denom - inv_denom)
+    int div_checked;     // the two-product quotient is exact for this divisor (two_product_quotient)
 };
 
 // Lower end of the dividends the FMA-corrected division handles (zero aside):
@@ -71,9 +73,19 @@ inline SweepArgs sweep_args(float* x, const float* b, int R, int P, int kind, fl
     if (kind == SWEEP_DIFFUSE) {
         A.inv_denom = 1.0f / denom;
         A.div_lo = division_floor(denom);
+        // 1 - d y is exact in double (d y has at most 48 significant bits)
+        A.inv_denom_lo = static_cast<float>((1.0 - static_cast<double>(denom) * A.inv_denom) / denom);
     }
     return A;
 }
+
+// Whether q = RN(x y + RN(x y_lo)), y and y_lo those of A, equals RN(x / d)
+// for every dividend the FMA-based quotients take; if so, sets
+// A.div_checked and raises A.div_lo to where RN(x y_lo) stays normal.
+// Decided once per divisor by trying every significand of one binade (the
+// quotients of the other binades are exact power-of-two scalings of those)
+// and cached.  Host only; a diffuse solve's first call costs about 10-20 ms.
+void two_product_quotient(SweepArgs& A);
 
 // Up to three independent solves with the same sweep count, relaxed by the
 // same level launches (e.g. the density, u and v diffuses of one step).

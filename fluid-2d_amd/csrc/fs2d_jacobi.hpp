@@ -10,28 +10,47 @@ namespace jac {
 
 __device__ __forceinline__ float sgn(bool neg, float x) { return neg ? -x : x; }
 
-// The divisor of a diffuse solve: d = 1 + 4a, y = RN(1/d), and lo, the
-// smallest nonzero |dividend| the FMA-corrected division takes (the host's
-// division_floor: max(2^-100, d 2^-124), or +inf when d > 2^100).
+// The divisor of a diffuse solve: d = 1 + 4a, y = RN(1/d), y_lo = RN(1/d - y)
+// (so y + y_lo is 1/d to about 2^-48), and lo, the smallest nonzero
+// |dividend| the FMA-based quotients take (the host's division_floor:
+// max(2^-100, d 2^-124), or +inf when d > 2^100; raised where the two-product
+// quotient is in use, see two_product_quotient).
 struct Divisor {
-    float d, y, lo;
+    float d, y, lo, y_lo;
 };
 
-__device__ __forceinline__ Divisor divisor_of(const SweepArgs& A) { return Divisor{A.denom, A.inv_denom, A.div_lo}; }
+__device__ __forceinline__ Divisor divisor_of(const SweepArgs& A) {
+    return Divisor{A.denom, A.inv_denom, A.div_lo, A.inv_denom_lo};
+}
 
-// x / d correctly rounded, given y = RN(1/d): q0 = RN(x y) is within ~1.5 ulp,
-// one FMA correction makes it faithful, and Markstein's step from a faithful
-// quotient (remainder exact by FMA) rounds correctly (P. Markstein, IBM J. R&D
-// 34(1), 1990; Muller et al., Handbook of Floating-Point Arithmetic), for
-// lo <= |x| <= 2^100.  Other dividends (zero, tiny, huge, inf, NaN) take the
-// IEEE division.  Pinned against exact rational arithmetic in
-// tests/test_division.py.
-__device__ __forceinline__ float div_rn(float x, Divisor v) {
+// Dividends outside [lo, 2^100] (zero, tiny, huge, inf, NaN) take the IEEE
+// division.  The test is wave-uniform: the common step, where no lane needs
+// it, passes one untaken branch.
+__device__ __forceinline__ float div_outside(float q, float x, Divisor v) {
     const float ax = fabsf(x);
-    if (!(ax >= v.lo && ax <= 0x1p100f)) return x / v.d;
-    const float q0 = x * v.y;
-    const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, v.d, x), v.y, q0);
-    return __builtin_fmaf(__builtin_fmaf(-q1, v.d, x), v.y, q1);
+    const bool out = !(ax >= v.lo && ax <= 0x1p100f);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(out) != 0, 0)) q = out ? x / v.d : q;
+    return q;
+}
+
+// x / d correctly rounded.  q1 = RN(x y + RN(x y_lo)) is within 2^-47
+// relative of x/d before its rounding, so faithful; Markstein's step from a
+// faithful quotient, its remainder exact by FMA, rounds correctly (P.
+// Markstein, IBM J. R&D 34(1), 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, FMA-based division), for lo <= |x| <= 2^100.
+// Pinned against exact rational arithmetic in tests/test_division.py.
+__device__ __forceinline__ float div_rn(float x, Divisor v) {
+    const float q1 = __builtin_fmaf(x, v.y, x * v.y_lo);
+    return div_outside(__builtin_fmaf(__builtin_fmaf(-q1, v.d, x), v.y, q1), x, v);
+}
+
+// x / d for a divisor whose two-product quotient q1 = RN(x y + RN(x y_lo))
+// the host has checked to be correctly rounded for every dividend in
+// [lo, 2^100] (Brisebarre, Muller and Raina, "Accelerating correctly rounded
+// floating-point division when the divisor is known in advance", IEEE Trans.
+// Computers 53(8), 2004; the check is two_product_quotient).
+__device__ __forceinline__ float div_rn_checked(float x, Divisor v) {
+    return div_outside(__builtin_fmaf(x, v.y, x * v.y_lo), x, v);
 }
 
 // The update of one cell; operand order as in src/fluid_solver_cpu.cpp:107-108
@@ -42,9 +61,13 @@ __device__ __forceinline__ float relax(int kind, float b, float n, float s, floa
 }
 
 // ===================== Gauss-Seidel tile hyperplane ======================
-template <int KIND>
+// CHECKED: the diffuse quotient is div_rn_checked (a divisor the host checked).
+template <int KIND, bool CHECKED = false>
 __device__ __forceinline__ float relax_k(float b, float n, float s, float w, float e, float a, Divisor dv) {
-    if (KIND == SWEEP_DIFFUSE) return div_rn(b + a * (((n + s) + w) + e), dv);
+    if (KIND == SWEEP_DIFFUSE) {
+        const float x = b + a * (((n + s) + w) + e);
+        return CHECKED ? div_rn_checked(x, dv) : div_rn(x, dv);
+    }
     return ((((b + e) + w) + s) + n) / 4.0f;
 }
 

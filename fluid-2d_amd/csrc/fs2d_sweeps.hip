@@ -36,6 +36,9 @@
 #include "fs2d_jacobi.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace fs2d {
@@ -264,25 +267,26 @@ struct UnitLds {
     static constexpr int NXS = XH * XP + GUARD, NBS = BH * BP + GUARD;
 };
 
-template <int TW, int M, int L, int KIND>
+// CHECKED (diffuse only): every field's divisor passed two_product_quotient.
+template <int TW, int M, int L, int KIND, bool CHECKED>
 __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, float* __restrict__ X,
                                             float* __restrict__ Bs);
 
-template <int TW, int M, int L, int KIND>
+template <int TW, int M, int L, int KIND, bool CHECKED>
 __global__ __launch_bounds__(64 * M) void k_gs_unit(SweepSet S, const int2* __restrict__ work) {
     // blocks [f * units, (f + 1) * units) relax field f of the set, unit by unit
     using U = UnitLds<TW, M>;
     __shared__ float X[U::NXS];
     __shared__ float Bs[U::NBS];
     const int f = blockIdx.x / S.units_per_field;
-    gs_unit_run<TW, M, L, KIND>(S.a[f], unit_of(S, work, blockIdx.x - f * S.units_per_field), X, Bs);
+    gs_unit_run<TW, M, L, KIND, CHECKED>(S.a[f], unit_of(S, work, blockIdx.x - f * S.units_per_field), X, Bs);
 }
 
 // One level launch of two solves pipelined together: blocks
 // [0, (count - 1) n_a) relax the diffuse fields 0 .. count - 2 of the set on the
 // units of work_a (n_a per field), the rest relax the pressure field count - 1
 // on the units of work_b.
-template <int TW, int M, int L>
+template <int TW, int M, int L, bool CHECKED>
 __global__ __launch_bounds__(64 * M) void k_gs_unit_mixed(SweepSet S, const int2* __restrict__ work_a, int n_a,
                                                           const int2* __restrict__ work_b) {
     using U = UnitLds<TW, M>;
@@ -291,13 +295,13 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit_mixed(SweepSet S, const int2
     const int b = blockIdx.x, nd = (S.count - 1) * n_a;
     if (b < nd) {
         const int f = b / n_a;
-        gs_unit_run<TW, M, L, SWEEP_DIFFUSE>(S.a[f], work_a[b - f * n_a], X, Bs);
+        gs_unit_run<TW, M, L, SWEEP_DIFFUSE, CHECKED>(S.a[f], work_a[b - f * n_a], X, Bs);
     } else {
-        gs_unit_run<TW, M, L, SWEEP_PRESSURE>(S.a[S.count - 1], work_b[b - nd], X, Bs);
+        gs_unit_run<TW, M, L, SWEEP_PRESSURE, false>(S.a[S.count - 1], work_b[b - nd], X, Bs);
     }
 }
 
-template <int TW, int M, int L, int KIND>
+template <int TW, int M, int L, int KIND, bool CHECKED>
 __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, float* __restrict__ X,
                                             float* __restrict__ Bs) {
     using U = UnitLds<TW, M>;
@@ -421,7 +425,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     const float nv = __int_as_float(
                         __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
                     const float wv = (r == s) ? w_halo : xprev;
-                    const float v = relax_k<KIND>(bb[q], nv, so[q], wv, e[q], a, dv);
+                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], wv, e[q], a, dv);
                     if (r <= s) X[x0 + q] = v;
                     // xprev is consumed only where this lane was active one
                     // tick earlier (as W) or lane r + 1 is active now (as N,
@@ -442,7 +446,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                 for (int q = 0; q < L; ++q) {
                     const float nv = __int_as_float(
                         __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    const float v = relax_k<KIND>(bb[q], nv, so[q], xprev, e[q], a, dv);
+                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
                     X[x0 + q] = v;
                     xprev = v;
                 }
@@ -458,7 +462,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                 for (int q = 0; q < L; ++q) {
                     const int s = s0 + q;
                     const float nv = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    const float v = relax_k<KIND>(bb[q], nv, so[q], xprev, e[q], a, dv);
+                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
                     if (r >= s - (TW - 1)) X[x0 + q] = v;
                     xprev = v;
                 }
@@ -526,7 +530,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     } else {
                         wv = (c == c_lo) ? w_first : xprev;
                     }
-                    const float v = relax_k<KIND>(bb[q], nv, sv, wv, ev, a, dv);
+                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, sv, wv, ev, a, dv);
                     if (c >= c_lo && c <= c_hi) X[x0 + q] = v;
                     xprev = v;
                 }
@@ -867,15 +871,108 @@ void gs_plan_destroy(GsPlan* plan) {
     plan->host_work = nullptr;
 }
 
+// ---- the two-product quotient check ----------------------------------------
+namespace {
+// The significands m of [1, 2) for which x = 2^E m gives
+// RN(x y + RN(x y_lo)) != RN(x / d) (every operation a float one).  Two
+// copies of one loop: with the FMA instruction, and through the C library.
+__attribute__((target("fma"))) long two_product_misses_hw(float d, float y, float yl, int E) {
+    const float scale = std::ldexp(1.0f, E);
+    long misses = 0;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t bits = 0x3f800000u | m;
+        float x;
+        std::memcpy(&x, &bits, 4);
+        x *= scale;
+        if (__builtin_fmaf(x, y, x * yl) != x / d) ++misses;
+    }
+    return misses;
+}
+
+long two_product_misses_soft(float d, float y, float yl, int E) {
+    const float scale = std::ldexp(1.0f, E);
+    long misses = 0;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t bits = 0x3f800000u | m;
+        float x;
+        std::memcpy(&x, &bits, 4);
+        x *= scale;
+        if (std::fma(x, y, x * yl) != x / d) ++misses;
+    }
+    return misses;
+}
+}  // namespace
+
+void two_product_quotient(SweepArgs& A) {
+    A.div_checked = 0;
+    if (A.kind != SWEEP_DIFFUSE) return;
+    const float d = A.denom, y = A.inv_denom, yl = A.inv_denom_lo;
+    if (!(d >= 1.f && d <= 0x1p100f)) return;
+    // From lo up, RN(x y_lo) and the quotient are normal, so every operation
+    // scales exactly with the dividend's power of two, and one binade,
+    // [2^E, 2^(E + 1)) with 2^E above lo's own floor, decides all of them.
+    float lo = A.div_lo;
+    int E = 0;
+    if (yl != 0.f) {
+        const double t = std::ldexp(1.0, -126) / std::fabs(static_cast<double>(yl));
+        if (!(t <= 0x1p99)) return;
+        float tf = static_cast<float>(t);
+        if (static_cast<double>(tf) < t) tf = std::nextafter(tf, 0x1p127f);
+        lo = std::max(lo, tf);
+        E = std::max(0, std::ilogb(tf) + 1);
+    }
+    if (!(lo <= 0x1p100f)) return;
+    static std::mutex mu;
+    static std::unordered_map<uint64_t, bool> exact;
+    uint32_t db, lb;
+    std::memcpy(&db, &d, 4);
+    std::memcpy(&lb, &yl, 4);
+    const uint64_t key = (static_cast<uint64_t>(db) << 32) | lb;
+    bool ok;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = exact.find(key);
+        if (it != exact.end()) {
+            ok = it->second;
+        } else {
+            const long misses = __builtin_cpu_supports("fma") ? two_product_misses_hw(d, y, yl, E)
+                                                              : two_product_misses_soft(d, y, yl, E);
+            ok = misses == 0;
+            exact.emplace(key, ok);
+        }
+    }
+    if (ok) {
+        A.div_checked = 1;
+        A.div_lo = lo;
+    }
+}
+
+// Whether fields [0, count) of a set all take the checked two-product quotient.
+static bool all_checked(const SweepSet& set, int count) {
+    for (int f = 0; f < count; ++f)
+        if (!set.a[f].div_checked) return false;
+    return count > 0;
+}
+
+// A copy of a set whose diffuse fields are marked by two_product_quotient.
+static SweepSet with_checked_divisions(const SweepSet& set) {
+    SweepSet out = set;
+    for (int f = 0; f < std::min(out.count, SweepSet::kMax); ++f)
+        if (out.a[f].kind == SWEEP_DIFFUSE) two_product_quotient(out.a[f]);
+    return out;
+}
+
 template <int TW, int M, int L>
 void launch_unit(hipStream_t s, int n, const int2* w, const SweepSet& set) {
     SweepSet level = set;
     level.units_per_field = n;
     const dim3 grid(n * set.count);
-    if (set.a[0].kind == SWEEP_DIFFUSE)
-        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_DIFFUSE>), grid, dim3(64 * M), 0, s, level, w);
+    if (set.a[0].kind != SWEEP_DIFFUSE)
+        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_PRESSURE, false>), grid, dim3(64 * M), 0, s, level, w);
+    else if (all_checked(set, set.count))
+        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_DIFFUSE, true>), grid, dim3(64 * M), 0, s, level, w);
     else
-        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_PRESSURE>), grid, dim3(64 * M), 0, s, level, w);
+        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_DIFFUSE, false>), grid, dim3(64 * M), 0, s, level, w);
 }
 
 template <int TW, int M>
@@ -890,8 +987,11 @@ bool launch_unit_lag(hipStream_t s, int lag, int n, const int2* w, const SweepSe
 
 template <int TW, int M, int L>
 void launch_mixed(hipStream_t s, int n_a, const int2* wa, int n_b, const int2* wb, const SweepSet& set) {
-    hipLaunchKernelGGL((k_gs_unit_mixed<TW, M, L>), dim3((set.count - 1) * n_a + n_b), dim3(64 * M), 0, s, set, wa,
-                       n_a, wb);
+    const dim3 grid((set.count - 1) * n_a + n_b);
+    if (all_checked(set, set.count - 1))
+        hipLaunchKernelGGL((k_gs_unit_mixed<TW, M, L, true>), grid, dim3(64 * M), 0, s, set, wa, n_a, wb);
+    else
+        hipLaunchKernelGGL((k_gs_unit_mixed<TW, M, L, false>), grid, dim3(64 * M), 0, s, set, wa, n_a, wb);
 }
 
 template <int TW, int M>
@@ -943,8 +1043,9 @@ bool launch_unit_any(hipStream_t s, int TW, int M, int lag, int n, const int2* w
     return false;
 }
 
-hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set) {
-    if (set.count < 1 || set.count > SweepSet::kMax) return hipErrorInvalidValue;
+hipError_t launch_gs_solve_set(hipStream_t s, const GsPlan& plan, const SweepSet& set_in) {
+    if (set_in.count < 1 || set_in.count > SweepSet::kMax) return hipErrorInvalidValue;
+    const SweepSet set = with_checked_divisions(set_in);
     for (int f = 0; f < set.count; ++f)
         if (plan.K != set.a[f].K || set.a[f].kind != set.a[0].kind || set.a[f].R != set.a[0].R)
             return hipErrorInvalidValue;
@@ -1009,6 +1110,7 @@ hipError_t launch_gs_solve_implicit(hipStream_t s, int R, int K, int TW, int M, 
     SweepSet set{};
     set.a[0] = a;
     set.count = 1;
+    set = with_checked_divisions(set);
     set.gi = (n + K - 1 + GS_ROWS - 1) / GS_ROWS;
     set.gj = (n + K - 1 + TW - 1) / TW;
     set.gk = (K + M - 1) / M;
@@ -1058,8 +1160,9 @@ int gs_pipeline_offset(const GsPlan& pp, int R, int Kd, int TWd, int Md) {
     return off;
 }
 
-hipError_t launch_gs_pipelined(hipStream_t s, const GsPlan& pd, const SweepSet& diffuse, const GsPlan& pp,
+hipError_t launch_gs_pipelined(hipStream_t s, const GsPlan& pd, const SweepSet& diffuse_in, const GsPlan& pp,
                                const SweepArgs& p, int offset) {
+    const SweepSet diffuse = with_checked_divisions(diffuse_in);
     if (pd.M == 0 || pp.M == 0 || pd.TW != pp.TW || pd.M != pp.M || pd.lag != pp.lag || !pd.level_of ||
         !pp.level_of || offset < 0 || pd.levels < 1 || pp.levels < 1)
         return hipErrorInvalidValue;

@@ -100,6 +100,7 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_time_diffuse_sweeps": (_i, [H, _f, _f, _f, _i, ctypes.POINTER(_f)]),
         "fs2d_hip_time_joint_sweeps": (_i, [H, _f, _f, _f, _i, ctypes.POINTER(_f)]),
         "fs2d_gs_pipeline_offset": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_i)]),
+        "fs2d_division_plan": (_i, [_f, ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_i)]),
         "fs2d_hip_gs_policy": (_i, [H, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
         "fs2d_hip_launches_per_step": (_i, [H, ctypes.POINTER(_i)]),
         "fs2d_hip_last_error": (ctypes.c_char_p, [H]),
@@ -303,7 +304,7 @@ EXPORTED_SYMBOLS = (
     "fs2d_hip_upload_device", "fs2d_hip_download_device", "fs2d_hip_stage_add_sources",
     "fs2d_hip_stage_diffuse", "fs2d_hip_stage_advect_density", "fs2d_hip_stage_advect_velocity",
     "fs2d_hip_stage_project", "fs2d_hip_stage_set_boundary", "fs2d_hip_download_pressure",
-    "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_time_diffuse_sweeps", "fs2d_hip_time_joint_sweeps", "fs2d_hip_gs_policy", "fs2d_gs_pipeline_offset", "fs2d_hip_launches_per_step",
+    "fs2d_hip_stream", "fs2d_hip_time_pressure_sweeps", "fs2d_hip_time_diffuse_sweeps", "fs2d_hip_time_joint_sweeps", "fs2d_hip_gs_policy", "fs2d_gs_pipeline_offset", "fs2d_division_plan", "fs2d_hip_launches_per_step",
     "fs2d_hip_last_error", "fs2d_hip_abi_version",
     "fs2d_hip_render_density", "fs2d_hip_render_density_device", "fs2d_hip_render_velocity",
     "fs2d_hip_render_velocity_device", "fs2d_hip_time_render", "fs2d_hip_project_noalloc",
@@ -420,6 +421,21 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
     if rc != 0:
         raise Fs2dError(f"fs2d_hip_copy_bandwidth failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
     return float(g.value)
+
+
+def division_plan(denom: float) -> tuple[float, float, float, bool]:
+    """The division of a diffuse solve by denom = 1 + 4a (host computation, no
+    device call): (inv, inv_lo, lo, checked) -- inv = RN(1/denom), inv_lo =
+    RN(1/denom - inv), lo the smallest nonzero |dividend| the FMA-based
+    quotients take, checked whether the two-product quotient alone is
+    correctly rounded from lo to 2^100 (what the Gauss-Seidel unit kernels use)."""
+    L = library()
+    inv, inv_lo, lo, chk = _f(), _f(), _f(), _i()
+    rc = L.fs2d_division_plan(float(denom), ctypes.byref(inv), ctypes.byref(inv_lo), ctypes.byref(lo),
+                              ctypes.byref(chk))
+    if rc != 0:
+        raise Fs2dError(f"fs2d_division_plan failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
+    return inv.value, inv_lo.value, lo.value, bool(chk.value)
 
 
 def gs_pipeline_offset(rows: int, diffuse_iterations: int, pressure_iterations: int, tile_cols: int,

@@ -178,6 +178,17 @@ int fs2d_hip_gs_policy(fs2d_handle* h, int kind, int* tile_cols, int* sweeps_per
  * the velocity its divergence reads. */
 int fs2d_gs_pipeline_offset(int rows, int diffuse_iterations, int pressure_iterations, int tile_cols,
                             int sweeps_per_unit, int* offset);
+/* Host only (no device call): the division a diffuse solve by denom = 1 + 4a
+ * uses (src/fluid_solver_cpu.cpp:107-108 divides by it in IEEE float).
+ * inv = RN(1 / denom), inv_lo = RN(1 / denom - inv); lo = the smallest
+ * nonzero |dividend| the FMA-based quotients take (smaller ones, zero, inf
+ * and NaN, and those above 2^100, take the IEEE division).  *checked = 1 when
+ * the two-product quotient RN(x inv + RN(x inv_lo)) was found correctly
+ * rounded for every dividend from lo to 2^100 (the Gauss-Seidel unit kernels
+ * then use it alone), else 0 (they add one Markstein correction, correctly
+ * rounded for every divisor).  The first call for a divisor tries 2^23
+ * dividends (about 10-20 ms); the answer is cached. */
+int fs2d_division_plan(float denom, float* inv, float* inv_lo, float* lo, int* checked);
 /* On-device copy bandwidth: a float4 copy kernel over `bytes` (read + write
  * counted), timed with HIP events over `reps` copies; the achievable HBM
  * figure beside the 8 TB/s peak. */

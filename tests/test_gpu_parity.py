@@ -240,6 +240,62 @@ def test_diffuse_extreme_values(R, rate, with_inf, order, block, gsb, lag, jk, o
                                              f"order={order} block={block} jk={jk}")
 
 
+def unchecked_rate(R, dt, start):
+    """A rate near `start` whose divisor 1 + 4 (dt R^2 rate) the two-product
+    quotient alone does not divide correctly for every dividend (the host's
+    check), so the unit kernels take the Markstein correction."""
+    for k in range(4000):
+        rate = np.float32(start * (1 + k / 997.0))
+        a = np.float32(dt) * np.float32(R * R) * rate
+        if not fs2d_amd.division_plan(float(np.float32(1) + np.float32(4) * a))[3]:
+            return float(rate)
+    raise AssertionError(f"no unchecked divisor near rate {start} at R={R}")
+
+
+@pytest.mark.parametrize("order,block,gsb,lag,jk", [v for v in GS_VARIANTS if v[2] > 1])
+@pytest.mark.parametrize("R", [67, 130, 258])
+def test_diffuse_unchecked_divisor(R, order, block, gsb, lag, jk, oracle_lib):
+    """The unit kernels' Markstein-corrected quotient, for divisors the
+    two-product check rejects (the other tests' divisors pass it)."""
+    rate = unchecked_rate(R, 0.1, 1e-3)
+    st = W.random_state(R, seed=5 * R)
+    for K in (1, 7, 20):
+        with make(R, st, order=order, block=block, gsb=gsb, lag=lag, jk=jk) as s:
+            for f in range(3):
+                s.diffuse(f, rate, 0.1, K)
+            got = s.download()
+        for f in range(3):
+            want = st[f].copy()
+            oracle_lib.diffuse(want, f, rate, 0.1, K, oracle_order(order))
+            assert_same(got[f], want, f"unchecked-divisor diffuse R={R} field={f} rate={rate} K={K} gs_block={gsb} "
+                                      f"lag={lag}")
+
+
+@pytest.mark.parametrize("gsb,lag", [(4, 4), (8, 4), (8, 8)])
+@pytest.mark.parametrize("which", ["density", "velocity"])
+def test_steps_unchecked_divisor(gsb, lag, which, oracle_lib):
+    """Whole steps (the diffuses and the first pressure solve in shared level
+    launches) with one diffuse divisor that the two-product check rejects."""
+    R = 130
+    diff, visc = 1e-4, 1e-5
+    if which == "density":
+        diff = unchecked_rate(R, 0.05, diff)
+    else:
+        visc = unchecked_rate(R, 0.05, 2e-4)
+    st = W.vortex_state(R, seed=3, n_vortices=16)
+    rs, us, vs = W.disc_sources(R, radius=R / 16)
+    want = [x.copy() for x in st]
+    with fs2d_amd.Solver(R, diffuse_iterations=12, pressure_iterations=30, order=GS, gs_block=gsb, gs_lag=lag) as s:
+        s.upload(*(x.copy() for x in st))
+        s.set_sources(rs, us, vs)
+        s.step_n(4, diff, visc, 0.05)
+        for _ in range(4):
+            oracle_lib.solve(want[0], rs, diff, want[1], want[2], us, vs, visc, 0.05, 12, 30, oracle.GS_LEX)
+        got = s.download()
+    for f, name in enumerate(("density", "u", "v")):
+        assert_same(got[f], want[f], f"{name} after 4 steps, unchecked {which} divisor, gs_block={gsb} lag={lag}")
+
+
 @pytest.mark.parametrize("order,block,gsb,lag,jk", GS_VARIANTS + [(JAC, 2, 0, 0, WAVE), (JAC, 4, 0, 0, WAVE),
                                                                   (JAC, 5, 0, 0, WAVE), (JAC, 8, 0, 0, WAVE),
                                                                   (JAC, 4, 0, 0, LDS)])
