@@ -318,8 +318,98 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
     const Ring rg = A.ring;
 
     // ---- load: iterate region and right-hand side in one batch, all loads in
-    // flight before any LDS store (one memory round trip per unit)
-    {
+    // flight before any LDS store (one memory round trip per unit).
+    if constexpr (KIND == SWEEP_PRESSURE) {
+        // By rows: the first TW columns of each region go as wave wv's rows
+        // wv, wv + M, ..., lane l's columns l, l + 64, ..., so the row (and its
+        // clamp into the grid) is wave-uniform and each load is a row pointer
+        // plus the lane's clamped column.  The narrow tail columns (M + 1 of
+        // the iterate region, M - 1 of the right-hand side) go cell by cell
+        // over all threads.
+        constexpr int CW = TW / 64;                                  // 64-column chunks of the first TW columns
+        constexpr int RX = (XH + M - 1) / M, RB = (BH + M - 1) / M;  // rows per wave
+        constexpr int XT = XW - TW, BT = BW - TW;                    // tail widths
+        constexpr int TX = (XH * XT + NT - 1) / NT, TB = (BH * BT + NT - 1) / NT;
+        static_assert(XT > 0 && BT > 0 && XT < 64 && BT < 64, "tails narrower than a chunk");
+        const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+        int jx[CW], jb[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            jx[c] = min(max(gj_lo + 64 * c + lane, 0), R - 1);
+            jb[c] = min(max(bj_lo + 64 * c + lane, 1), R - 2);
+        }
+        float v[RX][CW], vt[TX], vb[RB][CW], vbt[TB];
+#pragma unroll
+        for (int q = 0; q < RX; ++q) {
+            // rows past the region (last pass of some waves) repeat its last row, unstored
+            const int gi = min(max(gi_lo + min(wv + q * M, XH - 1), 0), R - 1);
+            const float* __restrict__ xr = x + (size_t)gi * P;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) v[q][c] = xr[jx[c]];
+        }
+#pragma unroll
+        for (int q = 0; q < TX; ++q) {
+            const int t = min(tid + q * NT, XH * XT - 1);
+            const int gi = min(max(gi_lo + t / XT, 0), R - 1), gj = min(max(gj_lo + TW + t % XT, 0), R - 1);
+            vt[q] = x[(size_t)gi * P + gj];
+        }
+        if (KIND == SWEEP_PRESSURE && A.du) {
+            // the divergence of (du, dv), src/fluid_solver_cpu.cpp:188-193
+            auto div = [&](const float* __restrict__ ur, const float* __restrict__ vn, const float* __restrict__ vs,
+                           int j) { return A.neg_half_h * (ur[j + 1] - ur[j - 1] + vs[j] - vn[j]); };
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const size_t o = (size_t)min(max(bi_lo + min(wv + q * M, BH - 1), 1), R - 2) * P;
+#pragma unroll
+                for (int c = 0; c < CW; ++c) vb[q][c] = div(A.du + o, A.dv + o - P, A.dv + o + P, jb[c]);
+            }
+#pragma unroll
+            for (int q = 0; q < TB; ++q) {
+                const int t = min(tid + q * NT, BH * BT - 1);
+                const size_t o = (size_t)min(max(bi_lo + t / BT, 1), R - 2) * P;
+                vbt[q] = div(A.du + o, A.dv + o - P, A.dv + o + P, min(max(bj_lo + TW + t % BT, 1), R - 2));
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const float* __restrict__ br = A.b + (size_t)min(max(bi_lo + min(wv + q * M, BH - 1), 1), R - 2) * P;
+#pragma unroll
+                for (int c = 0; c < CW; ++c) vb[q][c] = br[jb[c]];
+            }
+#pragma unroll
+            for (int q = 0; q < TB; ++q) {
+                const int t = min(tid + q * NT, BH * BT - 1);
+                const int gi = min(max(bi_lo + t / BT, 1), R - 2), gj = min(max(bj_lo + TW + t % BT, 1), R - 2);
+                vbt[q] = A.b[(size_t)gi * P + gj];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RX; ++q) {
+            const int row = wv + q * M;
+            if (row >= XH) break;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) X[row * XP + 64 * c + lane] = v[q][c];
+        }
+#pragma unroll
+        for (int q = 0; q < TX; ++q) {
+            const int t = tid + q * NT;
+            if (t < XH * XT) X[(t / XT) * XP + TW + t % XT] = vt[q];
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int row = wv + q * M;
+            if (row >= BH) break;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) Bs[row * BP + 64 * c + lane] = vb[q][c];
+        }
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int t = tid + q * NT;
+            if (t < BH * BT) Bs[(t / BT) * BP + TW + t % BT] = vbt[q];
+        }
+    } else {
+        // Cell by cell over the flattened regions (measured faster for the
+        // diffuse kernels than the row form above, slower for the pressure ones).
         float v[NX], vb[NBX];
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
@@ -327,22 +417,11 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
             const int gi = min(max(gi_lo + idx / XW, 0), R - 1), gj = min(max(gj_lo + idx % XW, 0), R - 1);
             v[q] = x[(size_t)gi * P + gj];
         }
-        if (KIND == SWEEP_PRESSURE && A.du) {
-            // the divergence of (du, dv), src/fluid_solver_cpu.cpp:188-193
 #pragma unroll
-            for (int q = 0; q < NBX; ++q) {
-                const int idx = min(tid + q * NT, BH * BW - 1);
-                const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
-                const size_t o = (size_t)gi * P + gj;
-                vb[q] = A.neg_half_h * (A.du[o + 1] - A.du[o - 1] + A.dv[o + P] - A.dv[o - P]);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < NBX; ++q) {
-                const int idx = min(tid + q * NT, BH * BW - 1);
-                const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
-                vb[q] = A.b[(size_t)gi * P + gj];
-            }
+        for (int q = 0; q < NBX; ++q) {
+            const int idx = min(tid + q * NT, BH * BW - 1);
+            const int gi = min(max(bi_lo + idx / BW, 1), R - 2), gj = min(max(bj_lo + idx % BW, 1), R - 2);
+            vb[q] = A.b[(size_t)gi * P + gj];
         }
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
@@ -399,9 +478,12 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
         // west halo column.  Waves w' <= w never write that cell (they would
         // reach it at a negative tile column); waves w' > w write it at tick
         // r - 1 + (2 + L) w' - 2 w, after this wave's step r (tick r + L w).
-        // So its value at the unit's start is the one step r reads: read it once.
+        // So its value at the unit's start is the one step r reads: read it
+        // once, into xprev, which a lane keeps until its first step (r), so
+        // that step takes it as W like every later step takes the lane's
+        // previous cell.
         __syncthreads();
-        const float w_halo = X[me0 + r - 1];
+        xprev = X[me0 + r - 1];
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
@@ -424,13 +506,10 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     const int s = s0 + q;
                     const float nv = __int_as_float(
                         __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    const float wv = (r == s) ? w_halo : xprev;
-                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], wv, e[q], a, dv);
-                    if (r <= s) X[x0 + q] = v;
-                    // xprev is consumed only where this lane was active one
-                    // tick earlier (as W) or lane r + 1 is active now (as N,
-                    // same column), so it need not be held across inactive ticks
-                    xprev = v;
+                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
+                    const bool act = (r <= s);
+                    if (act) X[x0 + q] = v;
+                    xprev = act ? v : xprev;
                 }
             } else if (s0 < TW) {
                 float nh[L];
@@ -549,16 +628,32 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
 
     // ---- write back every cell this unit relaxed (its latest sweep).  A unit
     // that meets the ring then writes the ring beside each of its edge cells
-    // (after the solve's last sweep that is the final ring), from LDS.
+    // (after the solve's last sweep that is the final ring), from LDS.  The
+    // stores are non-temporal: the next level reads them from beyond this
+    // XCD's L2 anyway, and fewer dirty L2 lines shorten the launch boundary.
     auto relaxed = [&](int gi, int gj) {
         const int lo = max(k_lo, max(GS_ROWS * I + 1 - gi, TW * J + 1 - gj));
         const int hi = min(k_hi, min(GS_ROWS * I + GS_ROWS - gi, TW * J + TW - gj));
         return gi >= 1 && gi <= n && gj >= 1 && gj <= n && lo <= hi;
     };
-    for (int idx = tid; idx < XH * XW; idx += NT) {
-        const int li = idx / XW, lj = idx % XW;
-        const int gi = gi_lo + li, gj = gj_lo + lj;
-        if (relaxed(gi, gj)) x[(size_t)gi * P + gj] = X[li * XP + lj];
+    {
+        // Region row li holds row gi of the sweeps k in [ka, kb] that relax
+        // it; those relax columns TW J + 1 - kb .. TW J + TW - ka (relaxed()
+        // for one row).  Rows are wave-uniform, as in the load.
+        constexpr int CX = (XW + 63) / 64;
+        const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+        for (int li = wv; li < XH; li += M) {
+            const int gi = gi_lo + li;
+            const int ka = max(k_lo, GS_ROWS * I + 1 - gi), kb = min(k_hi, GS_ROWS * I + GS_ROWS - gi);
+            if (gi < 1 || gi > n || ka > kb) continue;
+            const int j_lo = max(TW * J + 1 - kb, 1), j_hi = min(TW * J + TW - ka, n);
+            float* __restrict__ xr = x + (size_t)gi * P;
+#pragma unroll
+            for (int c = 0; c < CX; ++c) {
+                const int lj = 64 * c + lane, gj = gj_lo + lj;
+                if (lj < XW && gj >= j_lo && gj <= j_hi) __builtin_nontemporal_store(X[li * XP + lj], xr + gj);
+            }
+        }
     }
     if (!plain) {
         // rows 1 and n, then columns 1 and n, of this unit's region
@@ -574,12 +669,12 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
             if (row) {
                 const float tb = sgn(rg.neg_tb, v);
                 const size_t ring = far ? o + P : o - P;
-                x[ring] = tb;
+                __builtin_nontemporal_store(tb, x + ring);
                 const float cn = 0.5f * (tb + sgn(rg.neg_lr, v));  // the corners beside it
-                if (gj == 1) x[ring - 1] = cn;
-                if (gj == n) x[ring + 1] = cn;
+                if (gj == 1) __builtin_nontemporal_store(cn, x + ring - 1);
+                if (gj == n) __builtin_nontemporal_store(cn, x + ring + 1);
             } else {
-                x[far ? o + 1 : o - 1] = sgn(rg.neg_lr, v);
+                __builtin_nontemporal_store(sgn(rg.neg_lr, v), x + (far ? o + 1 : o - 1));
             }
         }
     }
