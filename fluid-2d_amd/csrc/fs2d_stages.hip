@@ -117,16 +117,54 @@ __global__ __launch_bounds__(BX* BY) void k_divergence(float* __restrict__ d, co
 // The gradient, with the largest finite |u|, |v| of the result reduced for
 // the next scatter: block reduction, then one atomic per block spread over
 // kSpeedSlots slots on separate cache lines (k_fold_speed folds them).
-__global__ __launch_bounds__(BX* BY) void k_gradient(float* __restrict__ u, float* __restrict__ v,
-                                                     const float* __restrict__ p, int R, int P, float h,
-                                                     int p_is_zero, unsigned* __restrict__ speed_slots) {
-    int i, j;
-    const bool in = interior(R, i, j);
+// Four cells per thread (columns j0 .. j0 + 3 of interior row i, j0 a multiple
+// of 4): u, v and the pressure rows as float4; cells::gradient's arithmetic
+// and ring writes per cell.  Quads that meet column 0 or pass column n take
+// the scalar body cell by cell.
+__global__ __launch_bounds__(256) void k_gradient(float* __restrict__ u, float* __restrict__ v,
+                                                  const float* __restrict__ p, int R, int P, float h,
+                                                  int p_is_zero, unsigned* __restrict__ speed_slots) {
+    const int j0 = 4 * (blockIdx.x * 256 + threadIdx.x), i = blockIdx.y + 1;
+    const int n = R - 2;
     float m = 0.f;
-    if (in) m = cells::gradient(u, v, p, R, P, h, p_is_zero != 0, i, j);
+    if (j0 >= 1 && j0 + 3 <= n) {
+        const size_t o = (size_t)i * P + j0;
+        const float4 uq = *reinterpret_cast<const float4*>(u + o), vq = *reinterpret_cast<const float4*>(v + o);
+        float pw[4] = {0.f, 0.f, 0.f, 0.f}, pe[4] = {0.f, 0.f, 0.f, 0.f};
+        float pn[4] = {0.f, 0.f, 0.f, 0.f}, ps[4] = {0.f, 0.f, 0.f, 0.f};
+        if (!p_is_zero) {
+            const float4 c = *reinterpret_cast<const float4*>(p + o);
+            const float4 sq = *reinterpret_cast<const float4*>(p + o + P);
+            const float4 nq = *reinterpret_cast<const float4*>(p + o - P);
+            pw[0] = p[o - 1]; pw[1] = c.x; pw[2] = c.y; pw[3] = c.z;
+            pe[0] = c.y; pe[1] = c.z; pe[2] = c.w; pe[3] = p[o + 4];
+            ps[0] = sq.x; ps[1] = sq.y; ps[2] = sq.z; ps[3] = sq.w;
+            pn[0] = nq.x; pn[1] = nq.y; pn[2] = nq.z; pn[3] = nq.w;
+        }
+        const float uo[4] = {uq.x, uq.y, uq.z, uq.w}, vo[4] = {vq.x, vq.y, vq.z, vq.w};
+        float un[4], vn[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            un[k] = uo[k] - 0.5f * (pe[k] - pw[k]) / h;
+            vn[k] = vo[k] - 0.5f * (ps[k] - pn[k]) / h;
+            m = cells::fold_speed(m, un[k], vn[k]);
+        }
+        *reinterpret_cast<float4*>(u + o) = make_float4(un[0], un[1], un[2], un[3]);
+        *reinterpret_cast<float4*>(v + o) = make_float4(vn[0], vn[1], vn[2], vn[3]);
+        if (i == 1 || i == n || j0 + 3 == n) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                cells::write_ring(u, i, j0 + k, un[k], R, P, Ring{false, true});
+                cells::write_ring(v, i, j0 + k, vn[k], R, P, Ring{true, false});
+            }
+        }
+    } else if (j0 < R) {
+        for (int j = max(j0, 1); j <= min(j0 + 3, n); ++j)
+            m = fmaxf(m, cells::gradient(u, v, p, R, P, h, p_is_zero != 0, i, j));
+    }
     if (speed_slots) {
         m = cells::block_max(m);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
+        if (threadIdx.x == 0) {
             const unsigned slot = (blockIdx.y * gridDim.x + blockIdx.x) % kSpeedSlots;
             atomicMax(speed_slots + slot * kSlotStride, __float_as_uint(m));
         }
@@ -245,7 +283,7 @@ hipError_t launch_divergence(hipStream_t s, float* d, const float* u, const floa
 
 hipError_t launch_gradient(hipStream_t s, float* u, float* v, const float* p, int R, int P, float h,
                            int p_is_zero, unsigned* max_bits, unsigned* speed_slots) {
-    hipLaunchKernelGGL(k_gradient, grid2d(R), dim3(BX, BY), 0, s, u, v, p, R, P, h, p_is_zero,
+    hipLaunchKernelGGL(k_gradient, dim3((R + 1023) / 1024, R - 2), dim3(256), 0, s, u, v, p, R, P, h, p_is_zero,
                        max_bits ? speed_slots : nullptr);
     if (max_bits) hipLaunchKernelGGL(k_fold_speed, dim3(1), dim3(kSpeedSlots), 0, s, max_bits, speed_slots);
     return hipGetLastError();
