@@ -1224,12 +1224,16 @@ int fs2d_gs_pipeline_offset(int rows, int diffuse_iterations, int pressure_itera
     return FS2D_OK;
 }
 
-int fs2d_division_plan(float denom, float* inv, float* inv_lo, float* lo, int* checked) {
-    if (!inv || !inv_lo || !lo || !checked) return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_division_plan: null output");
+int fs2d_division_plan(float denom, float* inv, float* inv_lo, float* two_hi, float* two_lo, float* lo,
+                       int* checked) {
+    if (!inv || !inv_lo || !two_hi || !two_lo || !lo || !checked)
+        return fail(nullptr, FS2D_ERR_ARGUMENT, "fs2d_division_plan: null output");
     SweepArgs A = sweep_args(nullptr, nullptr, 3, 64, SWEEP_DIFFUSE, 0.f, denom, BND_CONT, 1, 0);
     two_product_quotient(A);
     *inv = A.inv_denom;
     *inv_lo = A.inv_denom_lo;
+    *two_hi = A.inv_hi2;
+    *two_lo = A.inv_lo2;
     *lo = A.div_lo;
     *checked = A.div_checked;
     return FS2D_OK;

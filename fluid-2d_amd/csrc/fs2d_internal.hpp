@@ -56,6 +56,7 @@ struct SweepArgs {
     float neg_half_h;
     float inv_denom_lo;  // RN(1 / denom - inv_denom)
     int div_checked;     // the two-product quotient is exact for this divisor (two_product_quotient)
+    float inv_hi2, inv_lo2;  // its pair: inv_hi2 <= 1 / denom, inv_lo2 = RN(1 / denom - inv_hi2) > 0
 };
 
 // Lower end of the dividends the FMA-corrected division handles (zero aside):
@@ -79,12 +80,14 @@ inline SweepArgs sweep_args(float* x, const float* b, int R, int P, int kind, fl
     return A;
 }
 
-// Whether q = RN(x y + RN(x y_lo)), y and y_lo those of A, equals RN(x / d)
-// for every dividend the FMA-based quotients take; if so, sets
-// A.div_checked and raises A.div_lo to where RN(x y_lo) stays normal.
-// Decided once per divisor by trying every significand of one binade (the
-// quotients of the other binades are exact power-of-two scalings of those)
-// and cached.  Host only; a diffuse solve's first call costs about 10-20 ms.
+// Whether q = RN(x y2 + RN(x y2_lo)) equals RN(x / d) for every finite
+// dividend from the division floor up, with y2 the largest float <= 1/d and
+// y2_lo = RN(1/d - y2) > 0; if so, sets A.div_checked, A.inv_hi2 and
+// A.inv_lo2, and raises A.div_lo to where RN(x y2_lo) stays normal.  The tail
+// being positive, +-inf dividends come out +-inf as well.  Decided once per
+// divisor by trying every significand of one binade (the quotients of the
+// other binades are exact power-of-two scalings of those) and cached.  Host
+// only; a diffuse solve's first call with a new divisor costs about 4-18 ms.
 void two_product_quotient(SweepArgs& A);
 
 // Up to three independent solves with the same sweep count, relaxed by the

@@ -16,11 +16,11 @@ __device__ __forceinline__ float sgn(bool neg, float x) { return neg ? -x : x; }
 // max(2^-100, d 2^-124), or +inf when d > 2^100; raised where the two-product
 // quotient is in use, see two_product_quotient).
 struct Divisor {
-    float d, y, lo, y_lo;
+    float d, y, lo, y_lo, y2, y2_lo;
 };
 
 __device__ __forceinline__ Divisor divisor_of(const SweepArgs& A) {
-    return Divisor{A.denom, A.inv_denom, A.div_lo, A.inv_denom_lo};
+    return Divisor{A.denom, A.inv_denom, A.div_lo, A.inv_denom_lo, A.inv_hi2, A.inv_lo2};
 }
 
 // Dividends outside [lo, 2^100] (zero, tiny, huge, inf, NaN) take the IEEE
@@ -44,13 +44,18 @@ __device__ __forceinline__ float div_rn(float x, Divisor v) {
     return div_outside(__builtin_fmaf(__builtin_fmaf(-q1, v.d, x), v.y, q1), x, v);
 }
 
-// x / d for a divisor whose two-product quotient q1 = RN(x y + RN(x y_lo))
-// the host has checked to be correctly rounded for every dividend in
-// [lo, 2^100] (Brisebarre, Muller and Raina, "Accelerating correctly rounded
+// x / d for a divisor whose two-product quotient RN(x y2 + RN(x y2_lo)) the
+// host has checked to be correctly rounded for every finite dividend from lo
+// up (Brisebarre, Muller and Raina, "Accelerating correctly rounded
 // floating-point division when the divisor is known in advance", IEEE Trans.
-// Computers 53(8), 2004; the check is two_product_quotient).
+// Computers 53(8), 2004; the check is two_product_quotient).  With y2_lo > 0
+// an infinite dividend gives the infinity IEEE division gives, so one test,
+// |x| >= lo (false for zero, tiny, subnormal and NaN), guards it.
 __device__ __forceinline__ float div_rn_checked(float x, Divisor v) {
-    return div_outside(__builtin_fmaf(x, v.y, x * v.y_lo), x, v);
+    float q = __builtin_fmaf(x, v.y2, x * v.y2_lo);
+    const bool out = !(fabsf(x) >= v.lo);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(out) != 0, 0)) q = out ? x / v.d : q;
+    return q;
 }
 
 // The update of one cell; operand order as in src/fluid_solver_cpu.cpp:107-108

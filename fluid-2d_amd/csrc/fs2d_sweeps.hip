@@ -1001,22 +1001,26 @@ long two_product_misses_soft(float d, float y, float yl, int E) {
 void two_product_quotient(SweepArgs& A) {
     A.div_checked = 0;
     if (A.kind != SWEEP_DIFFUSE) return;
-    const float d = A.denom, y = A.inv_denom, yl = A.inv_denom_lo;
-    if (!(d >= 1.f && d <= 0x1p100f)) return;
-    // From lo up, RN(x y_lo) and the quotient are normal, so every operation
+    const float d = A.denom;
+    if (!(d > 1.f && d <= 0x1p100f)) return;
+    // y2: RN(1/d), or the float below it when that is above 1/d; 1 - d y2 is
+    // exact in double (d y2 has at most 48 significant bits) and positive
+    // unless 1/d is a float (then the plain quotient takes it).
+    float y = A.inv_denom;
+    if (1.0 - static_cast<double>(d) * y < 0.0) y = std::nextafter(y, 0.f);
+    const double rem = 1.0 - static_cast<double>(d) * y;
+    if (!(rem > 0.0)) return;
+    const float yl = static_cast<float>(rem / d);
+    if (!(yl > 0.f)) return;
+    // From lo up, RN(x y2_lo) and the quotient are normal, so every operation
     // scales exactly with the dividend's power of two, and one binade,
     // [2^E, 2^(E + 1)) with 2^E above lo's own floor, decides all of them.
-    float lo = A.div_lo;
-    int E = 0;
-    if (yl != 0.f) {
-        const double t = std::ldexp(1.0, -126) / std::fabs(static_cast<double>(yl));
-        if (!(t <= 0x1p99)) return;
-        float tf = static_cast<float>(t);
-        if (static_cast<double>(tf) < t) tf = std::nextafter(tf, 0x1p127f);
-        lo = std::max(lo, tf);
-        E = std::max(0, std::ilogb(tf) + 1);
-    }
-    if (!(lo <= 0x1p100f)) return;
+    const double t = std::ldexp(1.0, -126) / static_cast<double>(yl);
+    if (!(t <= 0x1p99)) return;
+    float tf = static_cast<float>(t);
+    if (static_cast<double>(tf) < t) tf = std::nextafter(tf, 0x1p127f);
+    const float lo = std::max(A.div_lo, tf);
+    const int E = std::max(0, std::ilogb(tf) + 1);
     static std::mutex mu;
     static std::unordered_map<uint64_t, bool> exact;
     uint32_t db, lb;
@@ -1039,6 +1043,8 @@ void two_product_quotient(SweepArgs& A) {
     if (ok) {
         A.div_checked = 1;
         A.div_lo = lo;
+        A.inv_hi2 = y;
+        A.inv_lo2 = yl;
     }
 }
 

@@ -100,7 +100,7 @@ def library() -> ctypes.CDLL:
         "fs2d_hip_time_diffuse_sweeps": (_i, [H, _f, _f, _f, _i, ctypes.POINTER(_f)]),
         "fs2d_hip_time_joint_sweeps": (_i, [H, _f, _f, _f, _i, ctypes.POINTER(_f)]),
         "fs2d_gs_pipeline_offset": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_i)]),
-        "fs2d_division_plan": (_i, [_f, ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_i)]),
+        "fs2d_division_plan": (_i, [_f] + [ctypes.POINTER(_f)] * 5 + [ctypes.POINTER(_i)]),
         "fs2d_hip_gs_policy": (_i, [H, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
         "fs2d_hip_launches_per_step": (_i, [H, ctypes.POINTER(_i)]),
         "fs2d_hip_last_error": (ctypes.c_char_p, [H]),
@@ -423,19 +423,22 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
     return float(g.value)
 
 
-def division_plan(denom: float) -> tuple[float, float, float, bool]:
+def division_plan(denom: float) -> dict:
     """The division of a diffuse solve by denom = 1 + 4a (host computation, no
-    device call): (inv, inv_lo, lo, checked) -- inv = RN(1/denom), inv_lo =
-    RN(1/denom - inv), lo the smallest nonzero |dividend| the FMA-based
-    quotients take, checked whether the two-product quotient alone is
-    correctly rounded from lo to 2^100 (what the Gauss-Seidel unit kernels use)."""
+    device call): inv = RN(1/denom) and inv_lo = RN(1/denom - inv), the pair of
+    the corrected quotient; two_hi <= 1/denom and two_lo = RN(1/denom - two_hi)
+    > 0, the pair of the two-product quotient alone; lo, the smallest nonzero
+    |dividend| the FMA-based quotients take; checked, whether the two-product
+    quotient alone is correctly rounded from lo up (what the Gauss-Seidel unit
+    kernels then use)."""
     L = library()
-    inv, inv_lo, lo, chk = _f(), _f(), _f(), _i()
-    rc = L.fs2d_division_plan(float(denom), ctypes.byref(inv), ctypes.byref(inv_lo), ctypes.byref(lo),
-                              ctypes.byref(chk))
+    out = [_f() for _ in range(5)]
+    chk = _i()
+    rc = L.fs2d_division_plan(float(denom), *(ctypes.byref(o) for o in out), ctypes.byref(chk))
     if rc != 0:
         raise Fs2dError(f"fs2d_division_plan failed ({rc}): {L.fs2d_hip_last_error(None).decode()}")
-    return inv.value, inv_lo.value, lo.value, bool(chk.value)
+    inv, inv_lo, two_hi, two_lo, lo = (o.value for o in out)
+    return {"inv": inv, "inv_lo": inv_lo, "two_hi": two_hi, "two_lo": two_lo, "lo": lo, "checked": bool(chk.value)}
 
 
 def gs_pipeline_offset(rows: int, diffuse_iterations: int, pressure_iterations: int, tile_cols: int,

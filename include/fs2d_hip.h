@@ -180,15 +180,19 @@ int fs2d_gs_pipeline_offset(int rows, int diffuse_iterations, int pressure_itera
                             int sweeps_per_unit, int* offset);
 /* Host only (no device call): the division a diffuse solve by denom = 1 + 4a
  * uses (src/fluid_solver_cpu.cpp:107-108 divides by it in IEEE float).
- * inv = RN(1 / denom), inv_lo = RN(1 / denom - inv); lo = the smallest
- * nonzero |dividend| the FMA-based quotients take (smaller ones, zero, inf
- * and NaN, and those above 2^100, take the IEEE division).  *checked = 1 when
- * the two-product quotient RN(x inv + RN(x inv_lo)) was found correctly
- * rounded for every dividend from lo to 2^100 (the Gauss-Seidel unit kernels
- * then use it alone), else 0 (they add one Markstein correction, correctly
- * rounded for every divisor).  The first call for a divisor tries 2^23
- * dividends (about 10-20 ms); the answer is cached. */
-int fs2d_division_plan(float denom, float* inv, float* inv_lo, float* lo, int* checked);
+ * inv = RN(1 / denom), inv_lo = RN(1 / denom - inv): the pair of the
+ * two-product quotient with one Markstein correction, correctly rounded for
+ * every divisor.  two_hi <= 1 / denom and two_lo = RN(1 / denom - two_hi) > 0:
+ * the pair of the two-product quotient alone.  *checked = 1 when that
+ * quotient, RN(x two_hi + RN(x two_lo)), was found correctly rounded for every
+ * finite dividend from lo up (the Gauss-Seidel unit kernels then use it; an
+ * infinite dividend comes out infinite too), else 0.  lo = the smallest
+ * nonzero |dividend| the FMA-based quotients take (smaller ones, zero and NaN,
+ * and for the corrected quotient those above 2^100 and inf, take the IEEE
+ * division).  The first call for a divisor tries 2^23 dividends (about 4-18
+ * ms); the answer is cached. */
+int fs2d_division_plan(float denom, float* inv, float* inv_lo, float* two_hi, float* two_lo, float* lo,
+                       int* checked);
 /* On-device copy bandwidth: a float4 copy kernel over `bytes` (read + write
  * counted), timed with HIP events over `reps` copies; the achievable HBM
  * figure beside the 8 TB/s peak. */

@@ -192,26 +192,43 @@ TWO_PRODUCT_CASES = [(float(d), True) for d in denominators()[:5]] + [
 ]
 
 
+def two_product_pair(d: np.float32):
+    """The pair of the two-product quotient as the host forms it: y2 = RN(1/d),
+    or the float below it when that is above 1/d, and y2_lo = RN((1 - d y2) / d)
+    (1 - d y2 exact in double), positive."""
+    y = np.float32(1.0) / d
+    if 1.0 - float(d) * float(y) < 0.0:
+        y = np.nextafter(y, np.float32(0.0))
+    return y, np.float32((1.0 - float(d) * float(y)) / float(d))
+
+
 @pytest.mark.parametrize("d,exact", TWO_PRODUCT_CASES, ids=lambda v: f"{v}")
 def test_host_two_product_check_agrees_with_enumeration(d, exact):
     import fs2d_amd
     d32 = np.float32(d)
-    inv, inv_lo_, lo, checked = fs2d_amd.division_plan(float(d32))
+    plan = fs2d_amd.division_plan(float(d32))
     y = np.float32(1.0) / d32
-    assert np.float32(inv) == y
-    assert np.float32(inv_lo_) == np.float32(float(inv_lo(d32, y)))
-    misses = binade_misses(d32, y, np.float32(inv_lo_))
+    assert np.float32(plan["inv"]) == y
+    assert np.float32(plan["inv_lo"]) == np.float32(float(inv_lo(d32, y)))
+    y2, y2l = two_product_pair(d32)
+    misses = binade_misses(d32, y2, y2l)
     assert (len(misses) == 0) == exact
-    assert checked == exact
-    assert lo >= division_floor(d32)
-    if checked:
-        # from lo up RN(x y_lo) is normal, so every binade scales the checked one
-        assert abs(float(np.float32(lo) * np.float32(inv_lo_))) >= 2.0 ** -126 or inv_lo_ == 0.0
+    assert plan["checked"] == exact
+    assert plan["lo"] >= division_floor(d32)
+    if plan["checked"]:
+        assert np.float32(plan["two_hi"]) == y2 and np.float32(plan["two_lo"]) == y2l and y2l > 0
+        assert Fraction(float(y2)) <= 1 / Fraction(float(d32))
+        # from lo up RN(x y2_lo) is normal, so every binade scales the checked one
+        assert float(np.float32(plan["lo"]) * y2l) >= 2.0 ** -126
+        # an infinite dividend: RN(inf y2 + RN(inf y2_lo)) = inf, as the IEEE division
+        with np.errstate(over="ignore", invalid="ignore"):
+            inf = np.float32(np.inf)
+            assert np.float32(inf * y2 + inf * y2l) == inf and np.float32(-inf * y2 + -inf * y2l) == -inf
     for x in misses[:4]:
         # where the two-product quotient misses, the Markstein step still rounds correctly
         X, D = Fraction(float(x)), Fraction(float(d32))
-        Y, YL = Fraction(float(y)), Fraction(float(np.float32(inv_lo_)))
-        assert two_product(X, Y, YL) != rn32(X / D)
+        Y, YL = Fraction(float(y)), Fraction(float(np.float32(plan["inv_lo"])))
+        assert two_product(X, Fraction(float(y2)), Fraction(float(y2l))) != rn32(X / D)
         assert markstein(X, D, Y, YL) == rn32(X / D)
 
 
@@ -220,8 +237,7 @@ def test_enumeration_is_exact_on_known_cases():
     significands, so that its two-sum and tie handling are pinned too."""
     rng = random.Random(7)
     for d in (np.float32(4.358721), np.float32(float.fromhex("0x1.a40e5ep+9")), np.float32(3.0)):
-        y = np.float32(1.0) / d
-        yl = np.float32(float(inv_lo(d, y)))
+        y, yl = two_product_pair(d)
         misses = {float(v) for v in binade_misses(d, y, yl)}
         D, Y, YL = Fraction(float(d)), Fraction(float(y)), Fraction(float(yl))
         ms = [rng.randrange(1 << 23) for _ in range(1500)] + [int(np.float32(v).view(np.uint32)) & 0x7FFFFF for v in misses]
@@ -231,10 +247,11 @@ def test_enumeration_is_exact_on_known_cases():
             assert (two_product(X, Y, YL) != rn32(X / D)) == (x in misses), (float(d), x)
 
 
-def test_two_product_check_needs_a_normal_tail():
-    """A divisor whose y_lo is zero (a power of two) is exact from the plain
-    division floor; the plan never lowers lo below it."""
+def test_two_product_check_needs_a_positive_tail():
+    """A divisor whose reciprocal is a float (a power of two, d = 1 when a = 0)
+    has no positive tail: the kernels keep the corrected quotient, exact there
+    too, and the plan leaves the division floor as it is."""
     import fs2d_amd
     for d in (1.0, 2.0, 1024.0):
-        inv, inv_lo_, lo, checked = fs2d_amd.division_plan(d)
-        assert inv_lo_ == 0.0 and checked and lo == division_floor(np.float32(d))
+        plan = fs2d_amd.division_plan(d)
+        assert plan["inv_lo"] == 0.0 and not plan["checked"] and plan["lo"] == division_floor(np.float32(d))

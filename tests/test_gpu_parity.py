@@ -247,7 +247,7 @@ def unchecked_rate(R, dt, start):
     for k in range(4000):
         rate = np.float32(start * (1 + k / 997.0))
         a = np.float32(dt) * np.float32(R * R) * rate
-        if not fs2d_amd.division_plan(float(np.float32(1) + np.float32(4) * a))[3]:
+        if not fs2d_amd.division_plan(float(np.float32(1) + np.float32(4) * a))["checked"]:
             return float(rate)
     raise AssertionError(f"no unchecked divisor near rate {start} at R={R}")
 
