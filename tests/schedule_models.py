@@ -750,3 +750,139 @@ def gs_pipeline_min_offset(R, Kd, Kp, TW, M):
                     last[np.ix_(np.unique(bi), np.unique(np.concatenate([bj - 1, bj + 1])))].max())
             need = max(need, int(m) + 1 - (I + J + kap))
     return need
+
+
+# ---- sheared units (a candidate schedule for the joint launches) -----------
+# In the coordinates i' = i + k, j'' = i + j + 2k every Gauss-Seidel dependence
+# of sweep k's cell (i, j) -- N (i - 1, j) and W (i, j - 1) of sweep k, S
+# (i + 1, j) and E (i, j + 1) of sweep k - 1 -- sits at i' - 1 or i', and at
+# j'' - 1.  So tiles of RW rows x TW columns x M sweeps in (i', j'', k) are
+# legal, units with equal I + J + kappa are independent, and inside a unit lane
+# r (row i' = RW I + 1 + r) relaxes j'' = TW J + 1 + s at step s: every lane is
+# active in every step (the rectangles in (i + k, j + k) leave lane r idle for
+# r of their RW + TW - 1 steps).  N comes from lane r - 1's previous step, W
+# from the lane's own; at step 0 lane r - 1's held west halo is lane r's north.
+
+def gs_shear_plan(R, K, TW, M, RW=64):
+    n = R - 2
+    GI = -(-(n + K - 1) // RW)
+    GK = -(-K // M)
+    levels = {}
+    for kap in range(GK):
+        k_lo, k_hi = kap * M, min(kap * M + M, K) - 1
+        for I in range(GI):
+            # j'' of band I's cells: (i' + 1 - k) + ... spans [RW I + 2, RW I + RW + n + 2 k_hi]
+            for J in range((RW * I + 1) // TW, (RW * I + RW + n + 2 * k_hi) // TW + 1):
+                if any(1 <= RW * I + 1 + r - k <= n and 1 <= TW * J + 1 + s - (RW * I + 1 + r) - k <= n
+                       for k in range(k_lo, k_hi + 1) for r in range(RW) for s in range(TW)):
+                    levels.setdefault(I + J + kap, []).append((I, J, kap))
+    return [levels[L] for L in sorted(levels)]
+
+
+def gs_shear_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, TW=64, RW=64):
+    """In-place Gauss-Seidel solve by sheared units, waves in lockstep (one
+    barrier per step); asserts the level and in-step conditions the unit
+    kernels rely on."""
+    R = x.shape[0]
+    n = R - 2
+    a, denom = F(a), F(denom)
+    for level in gs_shear_plan(R, K, TW, M, RW):
+        snap = x.copy()
+        reads, writes, updates = [], [], []
+        for (I, J, kap) in level:
+            k_lo, k_hi = kap * M, min(kap * M + M, K) - 1
+            lds = {}
+            rd = set()
+
+            def use(gi, gj):
+                assert 0 <= gi < R and 0 <= gj < R, (gi, gj)
+                if (gi, gj) not in lds:
+                    lds[(gi, gj)] = snap[gi, gj]
+                    rd.add((gi, gj))
+                return lds[(gi, gj)]
+
+            def cell(w, r, s):
+                k = k_lo + w
+                i = RW * I + 1 + r - k
+                return i, TW * J + 1 + s - (RW * I + 1 + r) - k
+
+            # each lane holds its west halo (step 0's W) before the first step
+            xprev = np.zeros((M, RW), np.float32)
+            held = np.ones((M, RW), bool)
+            for w in range(k_hi - k_lo + 1):
+                for r in range(RW):
+                    i, j = cell(w, r, 0)
+                    if 1 <= i <= n and 2 <= j <= n + 1:
+                        xprev[w, r] = use(i, j - 1)
+            last = {}
+            for s in range(TW):
+                pending, step_reads = [], set()
+                for w in range(k_hi - k_lo + 1):
+                    k = k_lo + w
+                    up = np.concatenate([[F(0)], xprev[w, :-1]])
+                    new_prev = xprev[w].copy()
+                    for r in range(RW):
+                        i, j = cell(w, r, s)
+                        if not (1 <= i <= n and 1 <= j <= n):
+                            continue
+                        zero0 = zero_start and k == 0
+                        self_old = use(i, j)
+                        if i == 1:
+                            N = _sgn(neg_tb, self_old) if k > 0 else (F(0) if zero_start else use(0, j))
+                        elif r == 0:
+                            N = use(i - 1, j)
+                        else:
+                            # lane r - 1's previous step (its held west halo at step 0)
+                            pi, pj = cell(w, r - 1, s - 1)
+                            assert (pi, pj) == (i - 1, j)
+                            N = up[r]
+                        if j == 1:
+                            Wv = _sgn(neg_lr, self_old) if k > 0 else (F(0) if zero_start else use(i, 0))
+                        else:
+                            Wv = xprev[w, r]
+                        if i == n:
+                            S = _sgn(neg_tb, self_old) if k > 0 else (F(0) if zero_start else use(n + 1, j))
+                        else:
+                            S = F(0) if zero0 else use(i + 1, j)
+                        if j == n:
+                            E = _sgn(neg_lr, self_old) if k > 0 else (F(0) if zero_start else use(i, n + 1))
+                        else:
+                            E = F(0) if zero0 else use(i, j + 1)
+                        step_reads.update({(i - 1, j), (i + 1, j), (i, j - 1), (i, j + 1)})
+                        v = _relax(kind, b[i, j], N, S, Wv, E, a, denom)
+                        pending.append((i, j, v))
+                        new_prev[r] = v
+                        last[(i, j)] = k
+                    xprev[w] = new_prev
+                wset = {(i, j) for (i, j, _) in pending}
+                assert len(wset) == len(pending)
+                assert not (step_reads & wset), ("in-step read/write overlap", s)
+                for (i, j, v) in pending:
+                    lds[(i, j)] = v
+            out = {}
+            for (gi, gj), khi in last.items():
+                v = lds[(gi, gj)]
+                out[(gi, gj)] = v
+                if khi == K - 1:
+                    tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                    cn = F(F(0.5) * F(tb + lr))
+                    if gi == 1: out[(0, gj)] = tb
+                    if gi == n: out[(n + 1, gj)] = tb
+                    if gj == 1: out[(gi, 0)] = lr
+                    if gj == n: out[(gi, n + 1)] = lr
+                    if gi == 1 and gj == 1: out[(0, 0)] = cn
+                    if gi == 1 and gj == n: out[(0, n + 1)] = cn
+                    if gi == n and gj == 1: out[(n + 1, 0)] = cn
+                    if gi == n and gj == n: out[(n + 1, n + 1)] = cn
+            reads.append(rd)
+            writes.append(set(out.keys()))
+            updates.append(out)
+        for t in range(len(level)):
+            for u in range(len(level)):
+                if t != u:
+                    assert not (reads[t] & writes[u]), ("read/write overlap", level[t], level[u])
+                    assert not (writes[t] & writes[u]), ("write/write overlap", level[t], level[u])
+        for out in updates:
+            for (i, j), val in out.items():
+                x[i, j] = val
+    return x

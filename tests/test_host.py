@@ -15,8 +15,8 @@ import fs2d_amd
 import oracle
 from conftest import ROOT, bits_equal
 from fs2d_amd import workloads as W
-from schedule_models import (gs_tile_model, gs_unit_lag_model, gs_unit_model, gs_unit_ring_model,
-                             jacobi_block_model, jacobi_wave_model)
+from schedule_models import (gs_shear_model, gs_shear_plan, gs_tile_model, gs_unit_lag_model, gs_unit_model,
+                             gs_unit_plan, gs_unit_ring_model, jacobi_block_model, jacobi_wave_model)
 
 
 def header_symbols():
@@ -133,6 +133,54 @@ def test_gs_unit_schedule_reproduces_gauss_seidel(R, M, oracle_lib):
     p = np.full((R, R), 7.0, np.float32)
     gs_unit_model(p, d, 9, 1, 0, 0, False, False, 1, M)
     assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M,TW,RW", [(3, 2, 4, 4), (10, 2, 4, 4), (23, 4, 8, 8), (40, 4, 8, 4), (34, 8, 16, 8)])
+def test_gs_shear_schedule_reproduces_gauss_seidel(R, M, TW, RW, oracle_lib):
+    """Sheared units (a candidate for the joint launches, not yet a kernel):
+    tiles of RW x TW x M in (i + k, i + j + 2k), every lane active in every
+    step, levels I + J + kappa; bit-identical to GS-lex, no two units of a
+    level touching a cell another writes, no in-step read of a written cell.
+    Small RW and TW give many tiles, edges and levels on a small grid."""
+    rho, u, v, *_ = W.random_state(R, seed=R + 2)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        for K in (1, 5, 9):
+            x = rho.copy()
+            gs_shear_model(x, rho.copy(), K, 0, a, den, mode == 2, mode == 1, 0, M, TW, RW)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert bits_equal(x, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
+    p = np.full((R, R), 7.0, np.float32)
+    gs_shear_model(p, d, 9, 1, 0, 0, False, False, 1, M, TW, RW)
+    assert bits_equal(p, p_ref)
+
+
+def test_gs_shear_plan_size_at_4096():
+    """The plan behind the sheared-unit estimate: 4096^2, K = 80, 64 x 128 x 8
+    units -- about as many units as the rectangles, about 30 % more levels."""
+    rect, shear = gs_unit_plan(4098, 80, 128, 8), gs_shear_plan_counts(4098, 80, 128, 8)
+    n_rect = sum(len(l) for l in rect)
+    assert len(rect) == 107 and n_rect == 21450
+    assert shear == (141, 22374)
+
+
+def gs_shear_plan_counts(R, K, TW, M, RW=64):
+    """(levels, units) of gs_shear_plan without testing every unit for an
+    interior cell (an upper bound on the units; the levels are the same)."""
+    n = R - 2
+    GI, GK = -(-(n + K - 1) // RW), -(-K // M)
+    levels, units = set(), 0
+    for kap in range(GK):
+        k_hi = min(kap * M + M, K) - 1
+        for I in range(GI):
+            for J in range((RW * I + 1) // TW, (RW * I + RW + n + 2 * k_hi) // TW + 1):
+                levels.add(I + J + kap)
+                units += 1
+    return len(levels), units
 
 
 @pytest.mark.parametrize("R,M,L", [(3, 2, 1), (10, 4, 4), (67, 2, 8), (70, 8, 8)])
