@@ -400,14 +400,14 @@ __global__ __launch_bounds__(1024) void k_sort_scan(unsigned* __restrict__ hist,
 }
 
 // Stable scatter of each chunk by digit: items of a chunk keep their order
-// within each digit (ranks by round, wave, then lane).
+// within each digit (ranks by pass, wave, then lane).
 template <class G>
 __global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned long long* __restrict__ in,
                                                      unsigned long long* __restrict__ out, long long N, int shift,
                                                      const unsigned* __restrict__ offs, int chunks, G gate) {
     if (!gate.sorted()) return;
-    __shared__ unsigned run[ND];           // items of each digit placed by earlier rounds
-    __shared__ unsigned wcnt[NT / 64][ND];  // items of each digit in each wave of this round
+    __shared__ unsigned run[ND];           // items of each digit placed by earlier passes
+    __shared__ unsigned wcnt[NT / 64][ND];  // items of each digit in each wave of this pass
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));  // lanes below this one
     for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
