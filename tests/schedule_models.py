@@ -886,3 +886,132 @@ def gs_shear_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, TW=64
             for (i, j), val in out.items():
                 x[i, j] = val
     return x
+
+
+def gs_shear_lag_model(x, b, K, kind, a, denom, neg_tb, neg_lr, zero_start, M, L, TW=64, RW=64):
+    """Sheared units with lagged waves, as k_gs_unit runs rectangles: wave w
+    runs its step s at tick s + L w, a barrier opens every interval of L ticks,
+    each wave reads every operand of its L steps right after it, and the
+    lane's held west halo is read once at the unit's start.  Asserts that no
+    operand a wave reads in an interval is written by another wave in the same
+    interval, and that no wave writes a held halo before its lane's step 0."""
+    R = x.shape[0]
+    n = R - 2
+    a, denom = F(a), F(denom)
+    TICKS = TW + L * (M - 1)
+    for level in gs_shear_plan(R, K, TW, M, RW):
+        snap = x.copy()
+        reads, writes, updates = [], [], []
+        for (I, J, kap) in level:
+            k_lo, k_hi = kap * M, min(kap * M + M, K) - 1
+            lds, rd = {}, set()
+
+            def use(gi, gj):
+                assert 0 <= gi < R and 0 <= gj < R, (gi, gj)
+                if (gi, gj) not in lds:
+                    lds[(gi, gj)] = snap[gi, gj]
+                    rd.add((gi, gj))
+                return lds[(gi, gj)]
+
+            def cell(w, r, s):
+                k = k_lo + w
+                return RW * I + 1 + r - k, TW * J + 1 + s - (RW * I + 1 + r) - k
+
+            nw = k_hi - k_lo + 1
+            xprev = np.zeros((M, RW), np.float32)
+            halo_until = {}  # held west halo cell -> tick of the lane's step 0
+            for w in range(nw):
+                for r in range(RW):
+                    i, j = cell(w, r, 0)
+                    if 1 <= i <= n and 2 <= j <= n + 1:
+                        xprev[w, r] = use(i, j - 1)
+                        halo_until[(i, j - 1)] = max(halo_until.get((i, j - 1), -1), L * w)
+            last, written_tick = {}, {}
+            for b0 in range(0, TICKS, L):
+                cap, cap_cells = {}, {w: set() for w in range(nw)}
+                for w in range(nw):
+                    for q in range(L):
+                        s = b0 + q - L * w
+                        if not 0 <= s < TW:
+                            continue
+                        for r in range(RW):
+                            i, j = cell(w, r, s)
+                            if not (1 <= i <= n and 1 <= j <= n):
+                                continue
+                            ops = {}
+                            for nm, (pi, pj) in (("self", (i, j)), ("n", (i - 1, j)), ("s", (i + 1, j)),
+                                                 ("w", (i, j - 1)), ("e", (i, j + 1))):
+                                ops[nm] = use(pi, pj)
+                                cap_cells[w].add((pi, pj))
+                            cap[(w, q, r)] = ops
+                wrote = {w: set() for w in range(nw)}
+                for q in range(L):
+                    for w in range(nw):
+                        k = k_lo + w
+                        s = b0 + q - L * w
+                        if not 0 <= s < TW:
+                            continue
+                        up = np.concatenate([[F(0)], xprev[w, :-1]])
+                        new_prev = xprev[w].copy()
+                        for r in range(RW):
+                            if (w, q, r) not in cap:
+                                continue
+                            i, j = cell(w, r, s)
+                            o = cap[(w, q, r)]
+                            zero0 = zero_start and k == 0
+                            so = o["self"]
+                            if i == 1:
+                                N = _sgn(neg_tb, so) if k > 0 else (F(0) if zero_start else o["n"])
+                            else:
+                                N = o["n"] if r == 0 else up[r]
+                            if j == 1:
+                                Wv = _sgn(neg_lr, so) if k > 0 else (F(0) if zero_start else o["w"])
+                            else:
+                                Wv = xprev[w, r]
+                            if i == n:
+                                S = _sgn(neg_tb, so) if k > 0 else (F(0) if zero_start else o["s"])
+                            else:
+                                S = F(0) if zero0 else o["s"]
+                            if j == n:
+                                E = _sgn(neg_lr, so) if k > 0 else (F(0) if zero_start else o["e"])
+                            else:
+                                E = F(0) if zero0 else o["e"]
+                            v = _relax(kind, b[i, j], N, S, Wv, E, a, denom)
+                            tick = s + L * w
+                            assert tick >= halo_until.get((i, j), -1), ("held halo overwritten", (i, j))
+                            lds[(i, j)] = v
+                            wrote[w].add((i, j))
+                            last[(i, j)] = k
+                            new_prev[r] = v
+                        xprev[w] = new_prev
+                for w in range(nw):
+                    for w2 in range(nw):
+                        if w != w2:
+                            assert not (cap_cells[w] & wrote[w2]), ("interval race", b0, w, w2)
+            out = {}
+            for (gi, gj), khi in last.items():
+                v = lds[(gi, gj)]
+                out[(gi, gj)] = v
+                if khi == K - 1:
+                    tb, lr = _sgn(neg_tb, v), _sgn(neg_lr, v)
+                    cn = F(F(0.5) * F(tb + lr))
+                    if gi == 1: out[(0, gj)] = tb
+                    if gi == n: out[(n + 1, gj)] = tb
+                    if gj == 1: out[(gi, 0)] = lr
+                    if gj == n: out[(gi, n + 1)] = lr
+                    if gi == 1 and gj == 1: out[(0, 0)] = cn
+                    if gi == 1 and gj == n: out[(0, n + 1)] = cn
+                    if gi == n and gj == 1: out[(n + 1, 0)] = cn
+                    if gi == n and gj == n: out[(n + 1, n + 1)] = cn
+            reads.append(rd)
+            writes.append(set(out.keys()))
+            updates.append(out)
+        for t in range(len(level)):
+            for u in range(len(level)):
+                if t != u:
+                    assert not (reads[t] & writes[u]), ("read/write overlap", level[t], level[u])
+                    assert not (writes[t] & writes[u]), ("write/write overlap", level[t], level[u])
+        for out in updates:
+            for (i, j), val in out.items():
+                x[i, j] = val
+    return x

@@ -15,7 +15,7 @@ import fs2d_amd
 import oracle
 from conftest import ROOT, bits_equal
 from fs2d_amd import workloads as W
-from schedule_models import (gs_shear_model, gs_shear_plan, gs_tile_model, gs_unit_lag_model, gs_unit_model,
+from schedule_models import (gs_shear_lag_model, gs_shear_model, gs_shear_plan, gs_tile_model, gs_unit_lag_model, gs_unit_model,
                              gs_unit_plan, gs_unit_ring_model, jacobi_block_model, jacobi_wave_model)
 
 
@@ -156,6 +156,30 @@ def test_gs_shear_schedule_reproduces_gauss_seidel(R, M, TW, RW, oracle_lib):
     p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
     p = np.full((R, R), 7.0, np.float32)
     gs_shear_model(p, d, 9, 1, 0, 0, False, False, 1, M, TW, RW)
+    assert bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("R,M,L,TW,RW", [(10, 2, 1, 4, 4), (23, 4, 2, 8, 8), (40, 4, 4, 8, 4), (34, 8, 4, 16, 8),
+                                        (34, 8, 8, 16, 8)])
+def test_gs_shear_lagged_schedule_reproduces_gauss_seidel(R, M, L, TW, RW, oracle_lib):
+    """Sheared units with lagged waves (a barrier every L ticks, every operand
+    of an interval read right after it, the held west halo read once at the
+    unit's start): bit-identical to GS-lex, no interval race, no held halo
+    overwritten before its lane's first step."""
+    rho, u, v, *_ = W.random_state(R, seed=R + 3)
+    a = np.float32(0.1) * np.float32(R * R) * np.float32(5e-2)
+    den = np.float32(1) + np.float32(4) * a
+    for mode in range(3):
+        for K in (1, 5, 9):
+            x = rho.copy()
+            gs_shear_lag_model(x, rho.copy(), K, 0, a, den, mode == 2, mode == 1, 0, M, L, TW, RW)
+            ref = rho.copy(); oracle_lib.diffuse(ref, mode, 5e-2, 0.1, K, oracle.GS_LEX)
+            assert bits_equal(x, ref), (mode, K)
+    d = oracle_lib.divergence(u, v)
+    aa, bb = u.copy(), v.copy()
+    p_ref = oracle_lib.project(aa, bb, 9, oracle.GS_LEX, want_p=True)
+    p = np.full((R, R), 7.0, np.float32)
+    gs_shear_lag_model(p, d, 9, 1, 0, 0, False, False, 1, M, L, TW, RW)
     assert bits_equal(p, p_ref)
 
 
