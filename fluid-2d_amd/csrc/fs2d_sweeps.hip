@@ -484,6 +484,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
         // previous cell.
         __syncthreads();
         xprev = X[me0 + r - 1];
+        const int guard_r = XH * XP + r;  // lane r's word in the guard space after X
         for (int it = 0; it < NINT; ++it) {
             __syncthreads();
             const int s0 = it * L - L * w;
@@ -508,7 +509,8 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                         __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
                     const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
                     const bool act = (r <= s);
-                    if (act) X[x0 + q] = v;
+                    // an idle lane stores to its own guard word (no exec mask change)
+                    X[(act ? x0 : guard_r - q) + q] = v;
                     xprev = act ? v : xprev;
                 }
             } else if (s0 < TW) {
@@ -542,7 +544,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     const int s = s0 + q;
                     const float nv = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(xprev), 0x138, 0xf, 0xf, false));
                     const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
-                    if (r >= s - (TW - 1)) X[x0 + q] = v;
+                    X[(r >= s - (TW - 1) ? x0 : guard_r - q) + q] = v;
                     xprev = v;
                 }
             }
