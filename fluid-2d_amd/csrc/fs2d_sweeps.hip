@@ -575,6 +575,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
         // no sweep writes -- read once (as in the plain units).
         __syncthreads();
         const float w_first = X[me0 + c_lo + r - 1];
+        const int guard_r = XH * XP + r;  // lane r's word in the guard space after X
         auto sweeps = [&](auto rows_t, auto cols_t) {
             constexpr bool RE = decltype(rows_t)::value, CE = decltype(cols_t)::value;
             for (int it = 0; it < NINT; ++it) {
@@ -612,7 +613,7 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                         wv = (c == c_lo) ? w_first : xprev;
                     }
                     const float v = relax_k<KIND, CHECKED>(bb[q], nv, sv, wv, ev, a, dv);
-                    if (c >= c_lo && c <= c_hi) X[x0 + q] = v;
+                    X[(c >= c_lo && c <= c_hi ? x0 : guard_r - q) + q] = v;
                     xprev = v;
                 }
             }
