@@ -613,7 +613,11 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                         wv = (c == c_lo) ? w_first : xprev;
                     }
                     const float v = relax_k<KIND, CHECKED>(bb[q], nv, sv, wv, ev, a, dv);
-                    X[(c >= c_lo && c <= c_hi ? x0 : guard_r - q) + q] = v;
+                    if constexpr (KIND == SWEEP_PRESSURE) {
+                        X[(c >= c_lo && c <= c_hi ? x0 : guard_r - q) + q] = v;
+                    } else {
+                        if (c >= c_lo && c <= c_hi) X[x0 + q] = v;
+                    }
                     xprev = v;
                 }
             }
