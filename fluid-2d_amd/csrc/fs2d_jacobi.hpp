@@ -76,6 +76,27 @@ __device__ __forceinline__ float relax_k(float b, float n, float s, float w, flo
     return ((((b + e) + w) + s) + n) / 4.0f;
 }
 
+// The same update with the diffuse quotient taken on the fast path only: the
+// wave's lanes whose dividend the fast path does not cover are ORed into bad
+// (a ballot, no branch), and the caller redoes its steps with relax_k when bad
+// is nonzero.  Pressure updates have no division and leave bad alone.
+template <int KIND, bool CHECKED>
+__device__ __forceinline__ float relax_fast(float b, float n, float s, float w, float e, float a, Divisor dv,
+                                            unsigned long long& bad) {
+    if (KIND == SWEEP_DIFFUSE) {
+        const float x = b + a * (((n + s) + w) + e);
+        if (CHECKED) {
+            bad |= __builtin_amdgcn_ballot_w64(!(fabsf(x) >= dv.lo));
+            return __builtin_fmaf(x, dv.y2, x * dv.y2_lo);
+        }
+        const float ax = fabsf(x);
+        bad |= __builtin_amdgcn_ballot_w64(!(ax >= dv.lo && ax <= 0x1p100f));
+        const float q1 = __builtin_fmaf(x, dv.y, x * dv.y_lo);
+        return __builtin_fmaf(__builtin_fmaf(-q1, dv.d, x), dv.y, q1);
+    }
+    return ((((b + e) + w) + s) + n) / 4.0f;
+}
+
 // ===================== Jacobi, register strips ===========================
 // One wave per tile and no LDS: lane l holds column gj0 + l of a 64-column
 // region, its TY + 2M rows in registers (iterate and right-hand side).  A

@@ -301,6 +301,16 @@ __global__ __launch_bounds__(64 * M) void k_gs_unit_mixed(SweepSet S, const int2
     }
 }
 
+// One cell update in the plain units' interval loop: exact (relax_k, the IEEE
+// division where the fast quotient does not cover the dividend) or fast
+// (relax_fast, recording such lanes in bad).
+template <int KIND, bool CHECKED, bool EXACT>
+__device__ __forceinline__ float relax_step(std::integral_constant<bool, EXACT>, float b, float n, float s, float w,
+                                            float e, float a, Divisor dv, unsigned long long& bad) {
+    if constexpr (EXACT) return relax_k<KIND, CHECKED>(b, n, s, w, e, a, dv);
+    else return relax_fast<KIND, CHECKED>(b, n, s, w, e, a, dv, bad);
+}
+
 template <int TW, int M, int L, int KIND, bool CHECKED>
 __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, float* __restrict__ X,
                                             float* __restrict__ Bs) {
@@ -492,6 +502,22 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
             if (!wave_ok || s0 + L <= 0 || s0 >= STEPS) continue;
             const int x0 = me0 + s0, b0 = bme0 + s0;
             float so[L], e[L], bb[L];
+            // An interval's steps take the fast quotient; if any lane's dividend
+            // needed the IEEE division, they are redone exactly from the
+            // interval's start (no other wave reads this interval's stores
+            // before the next barrier, so the first pass's stores are simply
+            // overwritten).  One branch per interval instead of one per step.
+            unsigned long long bad = 0;
+            auto run_interval = [&](auto steps) {
+                const float xp0 = xprev;
+                steps(std::false_type{});
+                if constexpr (KIND == SWEEP_DIFFUSE) {
+                    if (__builtin_expect(bad != 0, 0)) {
+                        xprev = xp0;
+                        steps(std::true_type{});
+                    }
+                }
+            };
             if (s0 < 64) {
                 float nh[L];
 #pragma unroll
@@ -502,17 +528,20 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     nh[q] = X[x0 + q - XP];
                 }
                 __builtin_amdgcn_sched_barrier(0);
+                auto steps = [&](auto exact_t) {
 #pragma unroll
-                for (int q = 0; q < L; ++q) {
-                    const int s = s0 + q;
-                    const float nv = __int_as_float(
-                        __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
-                    const bool act = (r <= s);
-                    // an idle lane stores to its own guard word (no exec mask change)
-                    X[(act ? x0 : guard_r - q) + q] = v;
-                    xprev = act ? v : xprev;
-                }
+                    for (int q = 0; q < L; ++q) {
+                        const int s = s0 + q;
+                        const float nv = __int_as_float(__builtin_amdgcn_update_dpp(
+                            __float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                        const float v = relax_step<KIND, CHECKED>(exact_t, bb[q], nv, so[q], xprev, e[q], a, dv, bad);
+                        const bool act = (r <= s);
+                        // an idle lane stores to its own guard word (no exec mask change)
+                        X[(act ? x0 : guard_r - q) + q] = v;
+                        xprev = act ? v : xprev;
+                    }
+                };
+                run_interval(steps);
             } else if (s0 < TW) {
                 float nh[L];
 #pragma unroll
@@ -523,14 +552,17 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     nh[q] = X[x0 + q - XP];
                 }
                 __builtin_amdgcn_sched_barrier(0);
+                auto steps = [&](auto exact_t) {
 #pragma unroll
-                for (int q = 0; q < L; ++q) {
-                    const float nv = __int_as_float(
-                        __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
-                    X[x0 + q] = v;
-                    xprev = v;
-                }
+                    for (int q = 0; q < L; ++q) {
+                        const float nv = __int_as_float(__builtin_amdgcn_update_dpp(
+                            __float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                        const float v = relax_step<KIND, CHECKED>(exact_t, bb[q], nv, so[q], xprev, e[q], a, dv, bad);
+                        X[x0 + q] = v;
+                        xprev = v;
+                    }
+                };
+                run_interval(steps);
             } else {
 #pragma unroll
                 for (int q = 0; q < L; ++q) {
@@ -539,14 +571,18 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     bb[q] = Bs[b0 + q];
                 }
                 __builtin_amdgcn_sched_barrier(0);
+                auto steps = [&](auto exact_t) {
 #pragma unroll
-                for (int q = 0; q < L; ++q) {
-                    const int s = s0 + q;
-                    const float nv = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, so[q], xprev, e[q], a, dv);
-                    X[(r >= s - (TW - 1) ? x0 : guard_r - q) + q] = v;
-                    xprev = v;
-                }
+                    for (int q = 0; q < L; ++q) {
+                        const int s = s0 + q;
+                        const float nv =
+                            __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                        const float v = relax_step<KIND, CHECKED>(exact_t, bb[q], nv, so[q], xprev, e[q], a, dv, bad);
+                        X[(r >= s - (TW - 1) ? x0 : guard_r - q) + q] = v;
+                        xprev = v;
+                    }
+                };
+                run_interval(steps);
             }
         }
     } else {
