@@ -629,32 +629,44 @@ __device__ __forceinline__ void gs_unit_run(const SweepArgs& A, const int2 wk, f
                     self[q] = X[x0 + q];
                 }
                 __builtin_amdgcn_sched_barrier(0);
+                // fast quotient for the interval, redone exactly if needed (as in the plain units)
+                unsigned long long bad = 0;
+                auto steps = [&](auto exact_t) {
 #pragma unroll
-                for (int q = 0; q < L; ++q) {
-                    const int c = s0 + q - r;
-                    // lane 0 receives the tile halo as the shift's old value
-                    const float up = __int_as_float(
-                        __builtin_amdgcn_update_dpp(__float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
-                    float nv = up, sv = so[q], ev = e[q], wv;
-                    if constexpr (RE) {
-                        const float tbv = __uint_as_float(__float_as_uint(self[q]) ^ tb_sign);
-                        nv = n_virtual ? tbv : (top ? nh[q] : up);
-                        sv = s_virtual ? tbv : so[q];
+                    for (int q = 0; q < L; ++q) {
+                        const int c = s0 + q - r;
+                        // lane 0 receives the tile halo as the shift's old value
+                        const float up = __int_as_float(__builtin_amdgcn_update_dpp(
+                            __float_as_int(nh[q]), __float_as_int(xprev), 0x138, 0xf, 0xf, false));
+                        float nv = up, sv = so[q], ev = e[q], wv;
+                        if constexpr (RE) {
+                            const float tbv = __uint_as_float(__float_as_uint(self[q]) ^ tb_sign);
+                            nv = n_virtual ? tbv : (top ? nh[q] : up);
+                            sv = s_virtual ? tbv : so[q];
+                        }
+                        if constexpr (CE) {
+                            const float lrv = __uint_as_float(__float_as_uint(self[q]) ^ lr_sign);
+                            wv = (c == c_lo) ? (w_virtual ? lrv : w_first) : xprev;
+                            ev = (kpos && c == c_right) ? lrv : e[q];
+                        } else {
+                            wv = (c == c_lo) ? w_first : xprev;
+                        }
+                        const float v = relax_step<KIND, CHECKED>(exact_t, bb[q], nv, sv, wv, ev, a, dv, bad);
+                        if constexpr (KIND == SWEEP_PRESSURE) {
+                            X[(c >= c_lo && c <= c_hi ? x0 : guard_r - q) + q] = v;
+                        } else {
+                            if (c >= c_lo && c <= c_hi) X[x0 + q] = v;
+                        }
+                        xprev = v;
                     }
-                    if constexpr (CE) {
-                        const float lrv = __uint_as_float(__float_as_uint(self[q]) ^ lr_sign);
-                        wv = (c == c_lo) ? (w_virtual ? lrv : w_first) : xprev;
-                        ev = (kpos && c == c_right) ? lrv : e[q];
-                    } else {
-                        wv = (c == c_lo) ? w_first : xprev;
+                };
+                const float xp0 = xprev;
+                steps(std::false_type{});
+                if constexpr (KIND == SWEEP_DIFFUSE) {
+                    if (__builtin_expect(bad != 0, 0)) {
+                        xprev = xp0;
+                        steps(std::true_type{});
                     }
-                    const float v = relax_k<KIND, CHECKED>(bb[q], nv, sv, wv, ev, a, dv);
-                    if constexpr (KIND == SWEEP_PRESSURE) {
-                        X[(c >= c_lo && c <= c_hi ? x0 : guard_r - q) + q] = v;
-                    } else {
-                        if (c >= c_lo && c <= c_hi) X[x0 + q] = v;
-                    }
-                    xprev = v;
                 }
             }
         };
