@@ -903,6 +903,31 @@ hipError_t launch_jacobi_block(hipStream_t s, const SweepArgs& a, const float* s
 
 }  // namespace
 
+// The pressure-only unit launches, built in a translation unit of their own
+// (fs2d_sweeps_p.hip, with LLVM's ILP-first machine scheduler, which times
+// faster for these kernels and slower for the diffuse and joint ones).
+bool launch_pressure_units(hipStream_t s, int TW, int M, int L, dim3 grid, const SweepSet& level,
+                           const int2* w)
+#ifdef FS2D_PRESSURE_UNITS_ONLY
+{
+#define FS2D_P(tw, m, l)                                                                         \
+    if (TW == tw && M == m && L == l) {                                                          \
+        hipLaunchKernelGGL((k_gs_unit<tw, m, l, SWEEP_PRESSURE, false>), grid, dim3(64 * m), 0, s, \
+                           level, w);                                                            \
+        return true;                                                                             \
+    }
+#define FS2D_PL(tw, m) FS2D_P(tw, m, 1) FS2D_P(tw, m, 4) FS2D_P(tw, m, 8)
+    FS2D_PL(64, 2) FS2D_PL(64, 4) FS2D_PL(64, 8) FS2D_PL(128, 4) FS2D_PL(128, 8)
+#undef FS2D_PL
+#undef FS2D_P
+    return false;
+}
+#else
+    ;
+#endif
+
+#ifndef FS2D_PRESSURE_UNITS_ONLY
+
 // ---- Gauss-Seidel plan ----------------------------------------------------
 hipError_t gs_plan_create(GsPlan* plan, int R, int K, int TW) {
     plan->TW = TW;
@@ -1124,7 +1149,7 @@ void launch_unit(hipStream_t s, int n, const int2* w, const SweepSet& set) {
     level.units_per_field = n;
     const dim3 grid(n * set.count);
     if (set.a[0].kind != SWEEP_DIFFUSE)
-        hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_PRESSURE, false>), grid, dim3(64 * M), 0, s, level, w);
+        launch_pressure_units(s, TW, M, L, grid, level, w);
     else if (all_checked(set, set.count))
         hipLaunchKernelGGL((k_gs_unit<TW, M, L, SWEEP_DIFFUSE, true>), grid, dim3(64 * M), 0, s, level, w);
     else
@@ -1396,5 +1421,7 @@ hipError_t launch_jacobi_solve(hipStream_t s, const SweepArgs& a, float* scratch
     }
     return hipSuccess;
 }
+
+#endif  // FS2D_PRESSURE_UNITS_ONLY
 
 }  // namespace fs2d
